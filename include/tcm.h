@@ -1,0 +1,212 @@
+/*
+ * tcm.h -- C ABI of libtcm, the MI355X (gfx950) dense statevector engine
+ * behind tensorcircuit_amd's Circuit API.
+ *
+ * The reference (tensorcircuit 0.12.1) has no native code: its statevector
+ * arithmetic is the tensordot loop of the contractor (cons.py:523-631,
+ * plain_contractor cons.py:334-370) reached from Circuit.wavefunction
+ * (circuit.py:792-810) and Circuit.expectation (circuit.py:914-990), with
+ * gradients from the torch backend's autograd (backends/pytorch_backend.py:
+ * 497-709).  Every entry point below replaces one of those Python-level
+ * operations; the replaced reference interface is cited on each.
+ *
+ * Conventions
+ *  - States are caller-owned device buffers of interleaved (re, im) complex
+ *    numbers, fp32 pairs for TCM_C64 and fp64 pairs for TCM_C128, laid out
+ *    [batch][2^n], qubit 0 = most significant bit of the amplitude index
+ *    (circuit.py:803-810).
+ *  - Parameters are device buffers [batch][nparams] of float (TCM_C64) or
+ *    double (TCM_C128).
+ *  - Energies, expectations and gradients are returned in fp64.
+ *  - All calls enqueue on `stream` (a hipStream_t, NULL = legacy default) and
+ *    never synchronise the device; results are valid once the stream is.
+ *  - Return value: 0 ok; < 0 argument error (TCM_E_*); > 0 a hipError_t.
+ *    tcm_last_error() gives a thread-local message for the last failure.
+ *  - The library allocates device memory only for program / Pauli-sum
+ *    descriptor tables (uploaded on first use per device, freed by
+ *    tcm_program_free / tcm_pauli_free) and never frees caller memory.
+ *    Scratch comes from the caller's workspace (tcm_*_workspace_bytes).
+ */
+#ifndef TCM_H_
+#define TCM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCM_ABI_VERSION 1
+
+enum tcm_dtype { TCM_C64 = 0, TCM_C128 = 1 };
+
+enum tcm_status {
+  TCM_OK = 0,
+  TCM_E_ARG = -1,       /* bad argument (null pointer, size, dtype) */
+  TCM_E_QUBIT = -2,     /* qubit index out of range */
+  TCM_E_DUPQUBIT = -3,  /* repeated qubit in one gate / term */
+  TCM_E_WORKSPACE = -4, /* workspace too small */
+  TCM_E_UNSUPPORTED = -5,
+  TCM_E_NODEVICE = -6,  /* no HIP device */
+  TCM_E_NONFINITE = -7  /* NaN / Inf in a host-side matrix or coefficient */
+};
+
+/* Gate kinds.  A "fixed" gate has param < 0 and uses `theta`; a trainable
+ * gate has param >= 0 and uses angle = scale * params[param].
+ * Matrices are complex, row-major, index[0] most significant
+ * (gates.py:248-336, backends/abstract_backend.py:390-400). */
+enum tcm_gate_kind {
+  TCM_G_MAT = 0,     /* constant 2^nq x 2^nq matrix in `mat` (1 <= nq <= 5) */
+  TCM_G_RX = 1,      /* exp(-i a/2 X)            gates.py:579-594 */
+  TCM_G_RY = 2,      /* exp(-i a/2 Y)            gates.py:600-615 */
+  TCM_G_RZ = 3,      /* exp(-i a/2 Z)            gates.py:621-636 */
+  TCM_G_RXX = 4,     /* exp(-i a/2 XX)           gates.py:863-865 */
+  TCM_G_RYY = 5,     /* exp(-i a/2 YY) */
+  TCM_G_RZZ = 6,     /* exp(-i a/2 ZZ) */
+  TCM_G_PHASE = 7,   /* diag(1, e^{i a})         gates.py:460-478 */
+  TCM_G_EXP1 = 8,    /* cos(a) I - i sin(a) G, G = `mat` with G^2 = I,
+                        nq in {1,2}              gates.py:825-858 */
+  TCM_G_BARRIER = 9  /* layer boundary (checkpoint candidate), no qubits */
+};
+
+typedef struct tcm_gate {
+  int32_t kind;        /* tcm_gate_kind */
+  int32_t nq;          /* number of qubits (0 for a barrier) */
+  int32_t q[5];        /* qubit indices, 0 <= q < nqubits, distinct */
+  int32_t param;       /* parameter slot, or -1 */
+  double theta;        /* fixed angle when param < 0 */
+  double scale;        /* angle = scale * params[param] */
+  const double* mat;   /* host complex matrix (re,im interleaved), or NULL */
+} tcm_gate;
+
+/* Engine options (all-zero = defaults). */
+typedef struct tcm_opts {
+  int32_t fuse;        /* 0: one kernel per gate ("plain");
+                          1: LDS-tiled passes, several gates per HBM pass */
+  int32_t tile_bits;   /* forward tile size log2 for fused passes; 0 = auto */
+  int32_t adj_tile_bits; /* adjoint tile size log2; 0 = auto */
+  int32_t ckpt_every;  /* adjoint: checkpoint the forward state every
+                          `ckpt_every` barriers (0 = none) */
+  int32_t reg_slots;   /* register-resident qubits per thread (3..5); 0 = auto */
+  int32_t reserved[3];
+} tcm_opts;
+
+typedef struct tcm_program tcm_program; /* opaque, caller-owned */
+typedef struct tcm_pauli tcm_pauli;     /* opaque, caller-owned */
+
+typedef struct tcm_program_info {
+  int32_t nqubits, nparams, ngates, npasses;
+  int32_t nckpt;              /* checkpoints stored by the adjoint */
+  int32_t tile_bits;          /* tile chosen for fused passes (0 = plain) */
+  int32_t reserved[2];
+} tcm_program_info;
+
+/* ---- library ------------------------------------------------------------ */
+
+int tcm_abi_version(void);
+const char* tcm_last_error(void);
+
+/* ---- gate programs --------------------------------------------------------
+ * Replaces the per-call graph building + contraction-path search of
+ * BaseCircuit.apply_general_gate (basecircuit.py:120-245) and the contractor
+ * (cons.py:634-663): the gate list (the reference's QIR, basecircuit.py:
+ * 131-142) is lowered once, fused into LDS-tiled passes and cached. */
+int tcm_program_build(const tcm_gate* gates, int32_t ngates, int32_t nqubits,
+                      int32_t nparams, const tcm_opts* opts, tcm_program** out);
+int tcm_program_free(tcm_program* prog);
+int tcm_program_get_info(const tcm_program* prog, tcm_program_info* info);
+
+/* Introspection (no reference counterpart): writes a JSON description of the
+ * schedule used for dtype (TCM_C64/TCM_C128) and mode (0 forward, 1 adjoint)
+ * into buf (NUL-terminated, truncated to cap); returns the full length. */
+int64_t tcm_program_describe(const tcm_program* prog, int32_t dtype,
+                             int32_t mode, char* buf, int64_t cap);
+
+/* Scratch bytes tcm_forward needs for `batch` states (per-item gate
+ * matrices evaluated from the parameters). */
+int tcm_forward_workspace_bytes(const tcm_program* prog, int32_t dtype,
+                                int64_t batch, int64_t* bytes);
+
+/* Forward simulation.  Replaces Circuit.wavefunction (circuit.py:792-810).
+ * If init_zero != 0 every state in the batch is first set to |0..0>
+ * (all_zero_nodes, basecircuit.py:47-60); otherwise `state` holds the input
+ * states (Circuit(n, inputs=...), circuit.py:86-96).  param_stride = number
+ * of parameter values between consecutive batch items (0 = shared).
+ * `ckpts` (nullable) receives info.nckpt x batch x 2^n amplitudes: the
+ * forward states at the checkpoint boundaries, for tcm_adjoint_vjp. */
+int tcm_forward(tcm_program* prog, int32_t dtype, const void* params,
+                int64_t param_stride, void* state, int64_t batch,
+                int32_t init_zero, void* ckpts, void* workspace,
+                int64_t ws_bytes, void* stream);
+
+/* ---- Pauli sums -----------------------------------------------------------
+ * codes: host int8 [nterms][nqubits], 0=I 1=X 2=Y 3=Z (quantum.py:1025-1068);
+ * coeffs: host real weights [nterms]. */
+int tcm_pauli_build(const int8_t* codes, const double* coeffs, int32_t nterms,
+                    int32_t nqubits, tcm_pauli** out);
+int tcm_pauli_free(tcm_pauli* ham);
+
+/* Scratch bytes for expectation / apply over `batch` states. */
+int tcm_pauli_workspace_bytes(const tcm_pauli* ham, int32_t dtype,
+                              int64_t batch, int64_t* bytes);
+
+/* out[b] = sum_k w_k <psi_b|P_k|psi_b> as complex fp64 (re, im) pairs,
+ * unnormalised.  Replaces Circuit.expectation_ps (abstractcircuit.py:
+ * 1208-1288) summed over terms by the caller (vqe_tc.py:75-81), with fp64
+ * accumulation instead of the reference's state-dtype sum. */
+int tcm_expect_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
+                         int64_t batch, double* out, void* workspace,
+                         int64_t ws_bytes, void* stream);
+
+/* out_state = alpha_b * (H psi_b), alpha given per batch item on the device
+ * (fp64 [batch], NULL = 1).  The vector-Jacobian product of expectation
+ * (backward of the reference's autograd through the expectation network,
+ * basecircuit.py:267-319). */
+int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
+                        void* out_state, int64_t batch, const double* alpha,
+                        void* stream);
+
+/* out[b] = <a_b|b_b> (complex fp64) -- expectation of general operators after
+ * they were applied to a copy of the state (circuit.py:914-990). */
+int tcm_inner_workspace_bytes(int32_t nqubits, int64_t batch, int64_t* bytes);
+int tcm_inner(int32_t nqubits, int32_t dtype, const void* a, const void* b,
+              int64_t batch, double* out, void* workspace, int64_t ws_bytes,
+              void* stream);
+
+/* ---- gradients ------------------------------------------------------------
+ * Replaces backend.value_and_grad / vvag over an energy function
+ * (backends/pytorch_backend.py:497-544, 673-709): one forward, E and
+ * lambda = H psi in fp64-accumulated kernels, then a reverse sweep
+ * (adjoint method) with forward checkpoints every opts.ckpt_every barriers.
+ * mode 0 = tcm_adjoint_energy_grad, mode 1 = tcm_adjoint_vjp. */
+int tcm_adjoint_workspace_bytes(const tcm_program* prog, const tcm_pauli* ham,
+                                int32_t dtype, int64_t batch, int32_t mode,
+                                int64_t* bytes);
+
+/* energy[b] = <psi_b|H|psi_b>, grad[b][p] = dE/dparams[b][p] (both fp64).
+ * `inputs` (nullable) are the initial states; NULL = |0..0>.  All states,
+ * checkpoints and partial sums live in the workspace. */
+int tcm_adjoint_energy_grad(tcm_program* prog, int32_t dtype,
+                            const void* params, int64_t param_stride,
+                            const tcm_pauli* ham, const void* inputs,
+                            int64_t batch, double* energy, double* grad,
+                            void* workspace, int64_t ws_bytes, void* stream);
+
+/* Vector-Jacobian product of the forward map params -> psi (torch
+ * convention, grad_state = dL/dRe psi + i dL/dIm psi):
+ * grad_params[b][p] = Re <grad_state_b | d psi_b / d params[b][p]> (fp64),
+ * grad_input (nullable) = U^dagger grad_state.  `state` holds the final
+ * states produced by tcm_forward with the same params (read only) and
+ * `ckpts` the checkpoints that call stored (required when info.nckpt > 0). */
+int tcm_adjoint_vjp(tcm_program* prog, int32_t dtype, const void* params,
+                    int64_t param_stride, const void* ckpts, const void* state,
+                    const void* grad_state, int64_t batch, double* grad_params,
+                    void* grad_input, void* workspace, int64_t ws_bytes,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TCM_H_ */

@@ -1,0 +1,436 @@
+"""``Circuit``: the reference's statevector Circuit API (circuit.py,
+basecircuit.py, abstractcircuit.py) on the HIP engine.
+
+Gate calls record the QIR exactly as the reference does
+(abstractcircuit.py:111-167: {"gatef", "index", "name", "split", "mpo",
+"parameters"}) and lower to engine gates; ``wavefunction`` / ``expectation``
+run the cached program on the device.  Gate methods are registered by
+reflection over the same name lists (abstractcircuit.py:217-326), including
+the upper-case and alias names and list/range fancy indexing.
+"""
+from __future__ import annotations
+
+from typing import Any, Dict, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+import torch
+
+from . import _lib, cons, engine
+from . import gates as G
+from .engine import GateSpec
+
+Tensor = Any
+
+
+def _is_sequence(x: Any) -> bool:
+    return isinstance(x, (list, tuple, range)) or (isinstance(x, np.ndarray) and x.ndim > 0) or (
+        isinstance(x, torch.Tensor) and x.dim() > 0)
+
+
+class _Op:
+    __slots__ = ("kind", "qubits", "theta", "scale", "mat")
+
+    def __init__(self, kind, qubits, theta=None, scale=1.0, mat=None):
+        self.kind, self.qubits, self.theta, self.scale, self.mat = kind, qubits, theta, scale, mat
+
+
+def _float_of(x: Any, what: str) -> float:
+    if isinstance(x, torch.Tensor):
+        try:
+            return float(x.detach().cpu().real.item())
+        except Exception as e:  # functorch-wrapped tensors cannot be read
+            raise NotImplementedError(
+                "%s: this gate is evaluated on the host and its angle cannot be a traced tensor; "
+                "use rx/ry/rz/rxx/ryy/rzz/phase/exp1 for trainable angles" % what) from e
+    return float(np.real(x))
+
+
+class Circuit:
+    """Statevector circuit on ``nqubits`` qubits (circuit.py:43-122).
+
+    ``inputs`` may hold 2^n or 2^(2n) amplitudes (circuit.py:86-96: the
+    latter lets a matrix be fed as a state, gates acting on the first n legs).
+    """
+
+    is_dm = False
+    sgates = G.sgates
+    vgates = G.vgates
+    gate_aliases = G.gate_aliases
+
+    def __init__(self, nqubits: int, inputs: Optional[Tensor] = None, mps_inputs: Any = None,
+                 split: Optional[Dict[str, Any]] = None) -> None:
+        if mps_inputs is not None:
+            raise NotImplementedError("mps_inputs: MPS simulation is outside this engine")
+        self._nqubits = int(nqubits)
+        self._qir: List[Dict[str, Any]] = []
+        self._extra_qir: List[Dict[str, Any]] = []
+        self._ops: List[_Op] = []
+        self.split = split
+        self.circuit_param = {"nqubits": nqubits, "inputs": inputs, "mps_inputs": None, "split": split}
+        self._nstate = self._nqubits
+        self.inputs = None
+        if inputs is not None:
+            t = inputs if isinstance(inputs, torch.Tensor) else torch.as_tensor(np.asarray(inputs))
+            t = t.reshape(-1)
+            N = t.shape[0]
+            m = int(round(np.log2(N)))
+            assert m == nqubits or m == 2 * nqubits
+            self._nstate = m
+            self.inputs = t
+        self.state_tensor = None
+
+    # ------------------------------------------------------------ recording --
+    def _norm_index(self, index: Sequence[int]) -> Tuple[int, ...]:
+        idx = tuple(int(i) if int(i) >= 0 else self._nqubits + int(i) for i in index)
+        assert len(idx) == len(set(idx))
+        for i in idx:
+            if i < 0 or i >= self._nqubits:
+                raise ValueError("qubit index %d out of range for %d qubits" % (i, self._nqubits))
+        return idx
+
+    def _record(self, op: _Op, ir: Dict[str, Any]) -> None:
+        self._ops.append(op)
+        self._qir.append(ir)
+        self.state_tensor = None  # refresh the state cache (basecircuit.py:245)
+
+    def apply_general_gate(self, gate: Any, *index: int, name: Optional[str] = None,
+                           split: Any = None, mpo: bool = False, ir_dict: Optional[Dict] = None) -> None:
+        """Apply an explicit gate matrix / Gate on ``index`` (basecircuit.py:120-245)."""
+        if mpo:
+            raise NotImplementedError("MPO gates are outside the dense statevector engine")
+        idx = self._norm_index(index)
+        m = G.matrix_of(gate)
+        if m.shape[0] != 2 ** len(idx):
+            raise ValueError("gate of dimension %d on %d qubits" % (m.shape[0], len(idx)))
+        ir = ir_dict or {"gate": gate, "index": idx, "name": name or "any", "split": split, "mpo": mpo}
+        self._record(_Op(_lib.G_MAT, idx, mat=m), ir)
+
+    apply = apply_general_gate
+
+    def _apply_named(self, gname: str, index: Tuple[int, ...], vars: Dict[str, Any], localname: str,
+                     split: Any) -> None:
+        idx = self._norm_index(index)
+        ir = {"gatef": getattr(G, gname), "index": idx, "target": list(idx), "name": localname,
+              "split": split, "mpo": False, "parameters": dict(vars)}
+        if gname in G.CONST:
+            self._record(_Op(_lib.G_MAT, idx, mat=np.asarray(G.CONST[gname], dtype=np.complex128)), ir)
+            return
+        if gname in G.TRAINABLE:
+            kind, scale = G.TRAINABLE[gname]
+            self._record(_Op(kind, idx, theta=vars.get("theta", 0.0), scale=scale), ir)
+            return
+        if gname in ("exp1", "exp"):
+            u = G.matrix_of(vars["unitary"])
+            theta = vars.get("theta", 0.0)
+            half = bool(vars.get("half", False)) if gname == "exp1" else False
+            if G.is_involution(u):
+                self._record(_Op(_lib.G_EXP1, idx, theta=theta, scale=0.5 if half else 1.0, mat=u), ir)
+            else:
+                th = _float_of(theta, gname)
+                m = G.exp1_matrix(u, th, half) if gname == "exp1" else G.exp_matrix(u, th)
+                self._record(_Op(_lib.G_MAT, idx, mat=m), ir)
+            return
+        if gname in ("any", "unitary"):
+            m = G.matrix_of(vars["unitary"])
+            self._record(_Op(_lib.G_MAT, idx, mat=m), ir)
+            return
+        if gname in G.HOST_PARAM:
+            f, keys = G.HOST_PARAM[gname]
+            kw = {k: _float_of(vars[k], gname) for k in keys if k in vars}
+            self._record(_Op(_lib.G_MAT, idx, mat=np.asarray(f(**kw), dtype=np.complex128)), ir)
+            return
+        raise ValueError("unknown gate %s" % gname)
+
+    @classmethod
+    def _make_sgate(cls, gname: str):
+        def apply(self, *index, split=None, name=None):
+            if isinstance(index[0], (int, np.integer)):
+                self._apply_named(gname, index, {}, name or gname, split)
+            elif _is_sequence(index[0]):
+                for ind in zip(*index):
+                    self._apply_named(gname, tuple(int(i) for i in ind), {}, name or gname, split)
+            else:
+                raise ValueError("Illegal index specification")
+
+        apply.__doc__ = "Apply **%s** gate (reference gates.py)." % gname.upper()
+        return apply
+
+    @classmethod
+    def _make_vgate(cls, gname: str):
+        def apply(self, *index, **vars):
+            split = vars.pop("split", None)
+            localname = vars.pop("name", gname)
+            if isinstance(index[0], (int, np.integer)):
+                self._apply_named(gname, index, vars, localname, split)
+            elif _is_sequence(index[0]):
+                for i, ind in enumerate(zip(*index)):
+                    nvars = {}
+                    for k, v in vars.items():
+                        try:
+                            nvars[k] = v[i]
+                        except Exception:  # scalar parameter shared by all
+                            nvars[k] = v
+                    self._apply_named(gname, tuple(int(j) for j in ind), nvars, localname, split)
+            else:
+                raise ValueError("Illegal index specification")
+
+        apply.__doc__ = "Apply **%s** gate with parameters (reference gates.py)." % gname.upper()
+        return apply
+
+    @classmethod
+    def _meta_apply(cls) -> None:
+        """abstractcircuit.py:217-326."""
+        for g in G.sgates:
+            setattr(cls, g, cls._make_sgate(g))
+            setattr(cls, g.upper(), cls._make_sgate(g))
+        for g in G.vgates:
+            setattr(cls, g, cls._make_vgate(g))
+            setattr(cls, g.upper(), cls._make_vgate(g))
+        for alias in G.gate_aliases:
+            for a in alias[1:]:
+                setattr(cls, a, getattr(cls, alias[0]))
+                setattr(cls, a.upper(), getattr(cls, alias[0]))
+
+    @staticmethod
+    def apply_general_variable_gate_delayed(gatef: Any, name: Optional[str] = None, mpo: bool = False):
+        """Register a user gate factory as a method (tests/test_gates.py:128-137);
+        the gate matrix is built on the host from the call's parameters."""
+        if name is None:
+            name = getattr(gatef, "n", "any")
+
+        def apply(self, *index, **vars):
+            localname = vars.pop("name", name)
+            vars.pop("split", None)
+            mat = gatef(**{k: (_float_of(v, name) if k != "unitary" else v) for k, v in vars.items()})
+            idx = self._norm_index(index)
+            ir = {"gatef": gatef, "index": idx, "target": list(idx), "name": localname, "split": None,
+                  "mpo": mpo, "parameters": vars}
+            self._record(_Op(_lib.G_MAT, idx, mat=G.matrix_of(mat)), ir)
+
+        return apply
+
+    def barrier_instruction(self, *index: Any) -> None:
+        """Layer marker; the adjoint engine may checkpoint the state here."""
+        self._record(_Op(_lib.G_BARRIER, ()), {"name": "barrier", "index": (), "gatef": None})
+
+    # ------------------------------------------------------------- lowering --
+    def _lower(self) -> Tuple[List[GateSpec], List[Any]]:
+        specs: List[GateSpec] = []
+        thetas: List[Any] = []
+        for op in self._ops:
+            if op.kind == _lib.G_BARRIER:
+                specs.append(engine.barrier())
+            elif op.kind == _lib.G_MAT:
+                specs.append(GateSpec(_lib.G_MAT, op.qubits, -1, 0.0, 1.0, op.mat))
+            else:
+                specs.append(GateSpec(op.kind, op.qubits, len(thetas), 0.0, op.scale, op.mat))
+                thetas.append(op.theta)
+        return specs, thetas
+
+    def _params(self, thetas: List[Any], dev: torch.device) -> torch.Tensor:
+        rdt = getattr(torch, cons.rdtypestr)
+        if not thetas:
+            return torch.zeros(0, dtype=rdt, device=dev)
+        if all(not isinstance(t, torch.Tensor) for t in thetas):
+            return torch.tensor([float(np.real(t)) for t in thetas], dtype=rdt, device=dev)
+        ts = []
+        for t in thetas:
+            if not isinstance(t, torch.Tensor):
+                t = torch.tensor(float(np.real(t)), dtype=rdt)
+            if t.is_complex():
+                t = torch.real(t)
+            ts.append(t.reshape(()).to(device=dev, dtype=rdt))
+        return torch.stack(ts)
+
+    def _program(self, specs: List[GateSpec], nparams: int) -> engine.Program:
+        return engine.get_program(self._nstate, specs, nparams, fuse=1)
+
+    def _run(self) -> torch.Tensor:
+        dev = engine.device()
+        specs, thetas = self._lower()
+        params = self._params(thetas, dev)
+        prog = self._program(specs, len(thetas))
+        inp = None
+        if self.inputs is not None:
+            inp = self.inputs.to(device=dev)
+            inp = inp.to(engine._CDT[cons.dtypestr])
+        st, _ = engine.StateFn.apply(params, inp, prog, cons.dtypestr)
+        return st
+
+    def _state(self, reuse: bool = True) -> torch.Tensor:
+        if reuse and self.state_tensor is not None:
+            return self.state_tensor
+        st = self._run()
+        if reuse:
+            self.state_tensor = st
+        return st
+
+    # -------------------------------------------------------------- outputs --
+    def wavefunction(self, form: str = "default") -> Tensor:
+        """circuit.py:792-810: "default" [-1], "ket" [-1, 1], "bra" [1, -1] (no conj)."""
+        s = self._state()
+        if form == "default":
+            return s.reshape(-1)
+        if form == "ket":
+            return s.reshape(-1, 1)
+        if form == "bra":
+            return s.reshape(1, -1)
+        raise ValueError("unknown form %r" % form)
+
+    state = wavefunction
+
+    def get_state(self) -> Tensor:
+        return self.wavefunction()
+
+    def amplitude(self, l: Union[str, Tensor]) -> Tensor:
+        """<l|psi> for a bitstring (basecircuit.py:447-508)."""
+        s = self._state()
+        if isinstance(l, str):
+            idx = int(l, 2)
+            return s[idx]
+        bits = torch.as_tensor(l).to(s.device)
+        weights = 2 ** torch.arange(self._nstate - 1, -1, -1, device=s.device)
+        return s[(bits.to(torch.int64) * weights).sum()]
+
+    def probability(self) -> Tensor:
+        """basecircuit.py:510-523."""
+        return torch.abs(self._state()) ** 2
+
+    def expectation_ps(self, x: Optional[Sequence[int]] = None, y: Optional[Sequence[int]] = None,
+                       z: Optional[Sequence[int]] = None, ps: Optional[Sequence[int]] = None,
+                       reuse: bool = True, noise_conf: Any = None, nmc: int = 1000,
+                       status: Any = None, **kws: Any) -> Tensor:
+        """Pauli-string expectation (abstractcircuit.py:1208-1288); ``ps``
+        (codes 0=I,1=X,2=Y,3=Z) overrides x/y/z."""
+        if noise_conf is not None:
+            raise NotImplementedError("noisy expectations are outside this engine")
+        if ps is not None:
+            from .quantum import ps2xyz
+
+            d = ps2xyz(ps)
+            x, y, z = d.get("x"), d.get("y"), d.get("z")
+        codes = np.zeros(self._nstate, dtype=np.int8)
+        seen = set()
+        for lst, code in ((x, 1), (y, 2), (z, 3)):
+            for i in lst or []:
+                i = int(i)
+                i = i if i >= 0 else self._nqubits + i
+                if i in seen:
+                    raise ValueError("Cannot measure two operators in one index")
+                seen.add(i)
+                codes[i] = code
+        return self._pauli_expectation(codes[None, :], np.ones(1), reuse)
+
+    def _pauli_expectation(self, codes: np.ndarray, coeffs: np.ndarray, reuse: bool = True) -> Tensor:
+        ham = engine.get_pauli(self._nstate, codes, coeffs)
+        return engine.PauliExpFn.apply(self._state(reuse), ham)
+
+    def expectation(self, *ops: Tuple[Any, List[int]], reuse: bool = True, enable_lightcone: bool = False,
+                    noise_conf: Any = None, nmc: int = 1000, status: Any = None, **kws: Any) -> Tensor:
+        """<psi|O_1...O_m|psi>, unnormalised, complex (circuit.py:914-990)."""
+        if noise_conf is not None:
+            raise NotImplementedError("noisy expectations are outside this engine")
+        occupied = set()
+        norm_ops = []
+        for op, index in ops:
+            if isinstance(index, (int, np.integer)):
+                index = [index]
+            idx = tuple(int(i) if int(i) >= 0 else self._nqubits + int(i) for i in index)
+            for i in idx:
+                if i in occupied:
+                    raise ValueError("Cannot measure two operators in one index")
+                occupied.add(i)
+            norm_ops.append((G.matrix_of(op), idx))
+        # Pauli strings take the fused fp64 kernel
+        codes = np.zeros(self._nstate, dtype=np.int8)
+        all_pauli = True
+        for m, idx in norm_ops:
+            c = G.pauli_code(m) if len(idx) == 1 else None
+            if c is None:
+                all_pauli = False
+                break
+            codes[idx[0]] = c
+        if all_pauli:
+            return self._pauli_expectation(codes[None, :], np.ones(1), reuse)
+        specs = [GateSpec(_lib.G_MAT, idx, -1, 0.0, 1.0, m) for m, idx in norm_ops]
+        dspecs = [GateSpec(_lib.G_MAT, idx, -1, 0.0, 1.0, m.conj().T) for m, idx in norm_ops]
+        prog = engine.get_program(self._nstate, specs, 0, fuse=1)
+        prog_dag = engine.get_program(self._nstate, dspecs, 0, fuse=1)
+        return engine.OpExpFn.apply(self._state(reuse), prog, prog_dag, cons.dtypestr)
+
+    # ------------------------------------------------------------ QIR / misc --
+    def to_qir(self) -> List[Dict[str, Any]]:
+        return self._qir
+
+    def gate_count(self, gate_list: Optional[Union[str, Sequence[str]]] = None) -> int:
+        """abstractcircuit.py gate_count."""
+        if gate_list is None:
+            return len([d for d in self._qir if d.get("name") != "barrier"])
+        if isinstance(gate_list, str):
+            gate_list = [gate_list]
+        names = set()
+        for g in gate_list:
+            names.add(g.lower())
+            for alias in G.gate_aliases:
+                if g.lower() in alias:
+                    names.update(alias)
+        return len([d for d in self._qir if str(d.get("name", "")).lower() in names])
+
+    def inverse(self) -> "Circuit":
+        """U^dagger as a new circuit (abstractcircuit.py:428-507)."""
+        c = Circuit(self._nqubits)
+        for op, ir in zip(reversed(self._ops), reversed(self._qir)):
+            if op.kind == _lib.G_BARRIER:
+                c.barrier_instruction()
+            elif op.kind == _lib.G_MAT:
+                c._record(_Op(_lib.G_MAT, op.qubits, mat=np.asarray(op.mat).conj().T), dict(ir))
+            else:
+                th = op.theta
+                c._record(_Op(op.kind, op.qubits, theta=-th if not isinstance(th, (int, float)) else -float(th),
+                              scale=op.scale, mat=op.mat), dict(ir))
+        return c
+
+    def matrix(self) -> Tensor:
+        """The circuit unitary (circuit.py matrix): columns U|j> for every j."""
+        n = self._nqubits
+        dev = engine.device()
+        specs, thetas = self._lower()
+        params = self._params(thetas, dev)
+        prog = engine.get_program(n, specs, len(thetas), fuse=1)
+        eye = torch.eye(2**n, dtype=engine._CDT[cons.dtypestr], device=dev)
+        p = params.reshape(1, -1).expand(2**n, -1).contiguous() if len(thetas) else None
+        st, _ = prog.forward(p, 2**n, cons.dtypestr, inputs=eye)
+        return st.T
+
+    def __repr__(self) -> str:
+        return "Circuit(nqubits=%d, gates=%d)" % (self._nqubits, len(self._qir))
+
+
+Circuit._meta_apply()
+
+
+def expectation(*ops: Tuple[Any, List[int]], ket: Tensor, bra: Optional[Tensor] = None,
+                conj: bool = True, normalization: bool = False) -> Tensor:
+    """Module-level <bra|O|ket> (circuit.py:997-1129)."""
+    dev = engine.device()
+    ket = torch.as_tensor(ket).to(device=dev).to(engine._CDT[cons.dtypestr]).reshape(-1)
+    n = int(round(np.log2(ket.shape[0])))
+    if bra is None:
+        bra = ket
+    else:
+        bra = torch.as_tensor(bra).to(device=dev).to(ket.dtype).reshape(-1)
+    specs = []
+    for op, index in ops:
+        if isinstance(index, (int, np.integer)):
+            index = [index]
+        idx = tuple(int(i) if int(i) >= 0 else n + int(i) for i in index)
+        specs.append(GateSpec(_lib.G_MAT, idx, -1, 0.0, 1.0, G.matrix_of(op)))
+    prog = engine.get_program(n, specs, 0, fuse=1)
+    phi, _ = prog.forward(None, 1, cons.dtypestr, inputs=ket.reshape(1, -1))
+    b = bra if conj else torch.conj(bra)
+    r = engine.inner(b.reshape(1, -1), phi, n).to(ket.dtype).reshape(())
+    if normalization:
+        nb = engine.inner(bra.reshape(1, -1), bra.reshape(1, -1), n).real
+        nk = engine.inner(ket.reshape(1, -1), ket.reshape(1, -1), n).real
+        r = r / torch.sqrt(nb * nk).to(r.dtype).reshape(())
+    return r

@@ -1,0 +1,654 @@
+// C ABI of libtcm (declared in include/tcm.h): argument checking, device
+// table upload, workspace carving and the launch sequences for forward
+// simulation, Pauli-sum energies and the adjoint reverse sweep.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace tcm;
+
+struct tcm_program {
+  Program p;
+};
+
+struct PauliDev {
+  void* terms = nullptr;
+};
+
+struct tcm_pauli {
+  int n = 0;
+  std::vector<DPGroup> groups;
+  std::vector<DPTerm> terms;
+  int nslots = 0;
+  std::mutex mu;
+  std::vector<PauliDev> dev;
+};
+
+namespace tcm {
+std::string describe_sched(const Program& P, int dtype, int mode);
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+}  // namespace tcm
+
+namespace {
+
+constexpr int64_t kAlign = 256;
+inline int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+int hip_fail(hipError_t e, const char* what) {
+  return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                        \
+  do {                                             \
+    hipError_t e_ = (expr);                        \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int current_device(int* dev) {
+  int cnt = 0;
+  hipError_t e = hipGetDeviceCount(&cnt);
+  if (e != hipSuccess || cnt <= 0) {
+    (void)hipGetLastError();
+    return set_error(TCM_E_NODEVICE, "no HIP device available");
+  }
+  HIP_TRY(hipGetDevice(dev), "hipGetDevice");
+  return 0;
+}
+
+template <typename V>
+int upload(const std::vector<V>& v, void** out) {
+  *out = nullptr;
+  if (v.empty()) return 0;
+  HIP_TRY(hipMalloc(out, v.size() * sizeof(V)), "hipMalloc(tables)");
+  HIP_TRY(hipMemcpy(*out, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice), "hipMemcpy(tables)");
+  return 0;
+}
+
+void free_tables(DeviceTables* D) {
+  if (!D) return;
+  void* ptrs[] = {D->dpass, D->dgroup, D->dop, D->dmem, D->ukbits, D->runoff, D->recipes, D->factors,
+                  D->gop_list, D->ctab, D->p_ptr, D->p_gops, D->gfac};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete D;
+}
+
+int ensure_tables(Program& P, int dtype, int mode, DeviceTables** out) {
+  int dev = 0;
+  if (int rc = current_device(&dev)) return rc;
+  std::lock_guard<std::mutex> lk(P.mu);
+  Sched& S = P.sched[dtype][mode];
+  if (S.k > 0 && S.dpass.size() != S.passes.size())
+    return set_error(TCM_E_ARG, "internal: schedule has %zu passes but %zu descriptors", S.passes.size(), S.dpass.size());
+  if ((int)S.dev.size() <= dev) S.dev.resize(dev + 1, nullptr);
+  if (S.dev[dev]) {
+    *out = S.dev[dev];
+    return 0;
+  }
+  DeviceTables* D = new DeviceTables();
+  int rc = 0;
+  rc = rc ? rc : upload(S.dpass, &D->dpass);
+  rc = rc ? rc : upload(S.dgroup, &D->dgroup);
+  rc = rc ? rc : upload(S.dop, &D->dop);
+  rc = rc ? rc : upload(S.dmem, &D->dmem);
+  rc = rc ? rc : upload(S.ukbits, &D->ukbits);
+  rc = rc ? rc : upload(S.runoff, &D->runoff);
+  rc = rc ? rc : upload(S.recipes, &D->recipes);
+  rc = rc ? rc : upload(S.factors, &D->factors);
+  rc = rc ? rc : upload(S.gop_list, &D->gop_list);
+  if (!rc) {
+    if (dtype == TCM_C64) {
+      std::vector<float> cf(P.ctab.begin(), P.ctab.end());
+      if (cf.empty()) cf.push_back(0.f);
+      rc = upload(cf, &D->ctab);
+    } else {
+      std::vector<double> cd(P.ctab);
+      if (cd.empty()) cd.push_back(0.0);
+      rc = upload(cd, &D->ctab);
+    }
+  }
+  rc = rc ? rc : upload(P.p_ptr, &D->p_ptr);
+  rc = rc ? rc : upload(P.p_gops, &D->p_gops);
+  rc = rc ? rc : upload(P.gfac, &D->gfac);
+  if (rc) {
+    free_tables(D);
+    return rc;
+  }
+  S.dev[dev] = D;
+  *out = D;
+  return 0;
+}
+
+inline size_t tsize(int dtype) { return dtype == TCM_C64 ? sizeof(float) : sizeof(double); }
+
+struct Carve {
+  char* base;
+  int64_t off = 0;
+  explicit Carve(void* b) : base((char*)b) {}
+  void* take(int64_t bytes) {
+    void* p = base ? base + off : nullptr;
+    off += align_up(bytes);
+    return p;
+  }
+};
+
+// ---- forward ---------------------------------------------------------------
+int64_t forward_ws(const Program& P, int dtype, int64_t batch) {
+  const Sched& S = P.sched[dtype][0];
+  return align_up(batch * (int64_t)std::max(S.mtab_size, 1) * (int64_t)tsize(dtype));
+}
+
+template <typename T>
+int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const void* mtab, void* state,
+                       int64_t batch, void* ckpts, hipStream_t st) {
+  const int n = P.n;
+  const int64_t N = 1ll << n;
+  const int64_t ck_stride = batch * N;
+  auto ckpt_ptr = [&](size_t c) { return (void*)((char*)ckpts + (int64_t)c * ck_stride * 2 * sizeof(T)); };
+  // checkpoints of the initial state (before any pass)
+  if (ckpts)
+    for (size_t c = 0; c < S.ckpt_after.size(); ++c)
+      if (S.ckpt_after[c] < 0) launch_copy<T>(ckpt_ptr(c), state, ck_stride, st);
+  const int np = (int)S.passes.size();
+  for (int pi = 0; pi < np; ++pi) {
+    const bool tiled = S.k > 0 && S.dpass[pi].plain_op < 0;
+    void* ckout = nullptr;
+    std::vector<size_t> extra;
+    if (ckpts)
+      for (size_t c = 0; c < S.ckpt_after.size(); ++c)
+        if (S.ckpt_after[c] == pi) {
+          if (!ckout && tiled) ckout = ckpt_ptr(c);
+          else extra.push_back(c);
+        }
+    if (tiled) {
+      if (S.R == 4) launch_fused_fwd<T, 4>(S, D, pi, mtab, D.ctab, state, batch, n, ckout, st);
+      else return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
+    } else {
+      const DOp& op = S.k > 0 ? S.dop[S.dpass[pi].plain_op] : S.dop[pi];
+      launch_plain_gate<T, false>(op, (const int32_t*)D.ukbits, mtab, S.mtab_size, D.ctab, state, n, batch, st);
+    }
+    for (size_t c : extra) launch_copy<T>(ckpt_ptr(c), state, ck_stride, st);
+  }
+  return 0;
+}
+
+template <typename T>
+int forward_impl(Program& P, int dtype, const void* params, int64_t pstride, void* state, int64_t batch,
+                 int init_zero, void* ckpts, void* ws, int64_t ws_bytes, hipStream_t st) {
+  DeviceTables* D = nullptr;
+  if (int rc = ensure_tables(P, dtype, 0, &D)) return rc;
+  const Sched& S = P.sched[dtype][0];
+  Carve cv(ws);
+  void* mtab = cv.take(batch * (int64_t)std::max(S.mtab_size, 1) * sizeof(T));
+  if (cv.off > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)cv.off);
+  launch_prep<T>(S, *D, D->ctab, params, pstride, mtab, batch, st);
+  if (init_zero) launch_init_zero<T>(state, P.n, batch, st);
+  if (int rc = run_forward_passes<T>(P, S, *D, mtab, state, batch, ckpts, st)) return rc;
+  HIP_TRY(hipGetLastError(), "forward launch");
+  return 0;
+}
+
+// ---- Pauli -----------------------------------------------------------------
+int ensure_pauli(tcm_pauli* H, const DPTerm** terms_dev) {
+  int dev = 0;
+  if (int rc = current_device(&dev)) return rc;
+  std::lock_guard<std::mutex> lk(H->mu);
+  if ((int)H->dev.size() <= dev) H->dev.resize(dev + 1);
+  if (!H->dev[dev].terms) {
+    if (int rc = upload(H->terms, &H->dev[dev].terms)) return rc;
+  }
+  *terms_dev = (const DPTerm*)H->dev[dev].terms;
+  return 0;
+}
+
+int pauli_blocks(int n, bool flip) {
+  const int64_t items = flip ? (1ll << (n - 1)) : (1ll << n);
+  int64_t b = items / 2048;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(1024, b));
+}
+
+int64_t pauli_ws(const tcm_pauli* H, int64_t batch) {
+  return align_up(batch * (int64_t)H->nslots * 2 * sizeof(double)) + align_up(batch * 2 * sizeof(double));
+}
+
+template <typename T>
+int pauli_energy_apply(tcm_pauli* H, const void* psi, void* out, const double* alpha, int64_t batch, double* epart,
+                       bool energy, bool apply, hipStream_t st) {
+  const DPTerm* td = nullptr;
+  if (int rc = ensure_pauli(H, &td)) return rc;
+  for (size_t g = 0; g < H->groups.size(); ++g)
+    launch_pauli<T>(H->groups[g], td, H->n, psi, out, g == 0 ? 1 : 0, alpha, epart, (int64_t)H->nslots * 2, batch,
+                    energy, apply, st);
+  return 0;
+}
+
+// ---- adjoint ---------------------------------------------------------------
+struct AdjWs {
+  void *mtab_f, *mtab_a, *psi, *lam, *ckpt;
+  double *part, *gsum, *epart, *e2;
+  int64_t bytes;
+};
+
+AdjWs carve_adj(const Program& P, const tcm_pauli* H, int dtype, int64_t batch, int mode, void* base) {
+  const Sched& F = P.sched[dtype][0];
+  const Sched& A = P.sched[dtype][1];
+  const int64_t N = 1ll << P.n;
+  const int64_t ts = tsize(dtype);
+  Carve cv(base);
+  AdjWs w{};
+  w.mtab_f = mode == 0 ? cv.take(batch * (int64_t)std::max(F.mtab_size, 1) * ts) : nullptr;
+  w.mtab_a = cv.take(batch * (int64_t)std::max(A.mtab_size, 1) * ts);
+  w.psi = cv.take(batch * N * 2 * ts);
+  w.lam = cv.take(batch * N * 2 * ts);
+  w.ckpt = mode == 0 ? cv.take((int64_t)P.ckpt_gate.size() * batch * N * 2 * ts) : nullptr;
+  const int64_t ngp = std::max(A.max_ngop_pass, 1);
+  w.part = (double*)cv.take(batch * ngp * (int64_t)A.max_slots * sizeof(double));
+  w.gsum = (double*)cv.take(batch * (int64_t)std::max(P.ngop, 1) * sizeof(double));
+  if (mode == 0 && H) {
+    w.epart = (double*)cv.take(batch * (int64_t)H->nslots * 2 * sizeof(double));
+    w.e2 = (double*)cv.take(batch * 2 * sizeof(double));
+  } else {
+    w.epart = w.e2 = nullptr;
+  }
+  w.bytes = cv.off;
+  return w;
+}
+
+template <typename T>
+int reverse_sweep(Program& P, int dtype, const AdjWs& w, const void* psi_src, const void* lam_src, const void* ckpts,
+                  void* grad_input, int64_t batch, hipStream_t st) {
+  DeviceTables* D = nullptr;
+  if (int rc = ensure_tables(P, dtype, 1, &D)) return rc;
+  const Sched& A = P.sched[dtype][1];
+  const int n = P.n;
+  const int64_t N = 1ll << n;
+  const int64_t tot = batch * N;
+  auto ckpt_ptr = [&](size_t c) { return (const void*)((const char*)ckpts + (int64_t)c * tot * 2 * sizeof(T)); };
+  const int64_t ngp = std::max(A.max_ngop_pass, 1);
+  const int64_t pstride = ngp * (int64_t)A.max_slots;
+  const int np = (int)A.passes.size();
+  // checkpoint restoring psi before pass p: the one taken after pass p-1
+  auto ckpt_before = [&](int p) -> int {
+    for (size_t c = 0; c < A.ckpt_after.size(); ++c)
+      if (A.ckpt_after[c] == p - 1) return (int)c;
+    return -1;
+  };
+  if (A.k == 0) {
+    // plain: work in place on the workspace copies
+    if (psi_src != w.psi) launch_copy<T>(w.psi, psi_src, tot, st);
+    if (lam_src != w.lam) launch_copy<T>(w.lam, lam_src, tot, st);
+    for (int p = np - 1; p >= 0; --p) {
+      const DOp& op = A.dop[p];
+      if (op.kind == OK_UK) {
+        launch_plain_adj_uk<T>(op, (const int32_t*)D->ukbits, D->ctab, w.psi, w.lam, batch, n, st);
+      } else {
+        const int slots = op.gop >= 0 ? A.gop_slots[op.gop] : std::min<int64_t>(512, std::max<int64_t>(1, (N >> 2) / 256));
+        launch_plain_adj<T>(op, w.mtab_a, A.mtab_size, D->ctab, w.psi, w.psi, w.lam, w.lam, batch, n, slots, w.part,
+                            pstride, st);
+        if (op.gop >= 0) launch_reduce_gop(w.part, pstride, slots, nullptr, op.gop, 1, w.gsum, P.ngop, batch, st);
+      }
+      int c = ckpt_before(p);
+      if (c >= 0 && ckpts) launch_copy<T>(w.psi, ckpt_ptr(c), tot, st);
+    }
+    if (grad_input) launch_copy<T>(grad_input, w.lam, tot, st);
+  } else {
+    const void* pin = psi_src;
+    const void* lin = lam_src;
+    for (int p = np - 1; p >= 0; --p) {
+      const DPass& dp = A.dpass[p];
+      int c = ckpts ? ckpt_before(p) : -1;
+      if (dp.plain_op >= 0) {
+        // >=3-qubit constant gate: undo in place on the workspace copies
+        if (pin != w.psi) launch_copy<T>(w.psi, pin, tot, st);
+        if (lin != w.lam) launch_copy<T>(w.lam, lin, tot, st);
+        launch_plain_adj_uk<T>(A.dop[dp.plain_op], (const int32_t*)D->ukbits, D->ctab, w.psi, w.lam, batch, n, st);
+        if (p == 0 && grad_input) launch_copy<T>(grad_input, w.lam, tot, st);
+        pin = c >= 0 ? ckpt_ptr(c) : w.psi;
+        lin = w.lam;
+        continue;
+      }
+      void* pout = c >= 0 ? nullptr : w.psi;
+      void* lout = (p == 0 && grad_input) ? grad_input : w.lam;
+      if (A.R == 4)
+        launch_fused_adj<T, 4>(A, *D, p, w.mtab_a, D->ctab, pin, pout, lin, lout, batch, n, w.part, pstride, st);
+      else
+        return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", A.R);
+      if (dp.ngop > 0)
+        launch_reduce_gop(w.part, pstride, 1ll << dp.nblk_log, (const int32_t*)D->gop_list + dp.gop_list0, 0,
+                          dp.ngop, w.gsum, P.ngop, batch, st);
+      pin = c >= 0 ? ckpt_ptr(c) : w.psi;
+      lin = lout;
+    }
+    if (np == 0 && grad_input) launch_copy<T>(grad_input, lam_src, tot, st);
+  }
+  return 0;
+}
+
+template <typename T>
+int energy_grad_impl(Program& P, int dtype, const void* params, int64_t pstride, tcm_pauli* H, const void* inputs,
+                     int64_t batch, double* energy, double* grad, void* ws, int64_t ws_bytes, hipStream_t st) {
+  AdjWs w = carve_adj(P, H, dtype, batch, 0, nullptr);
+  if (w.bytes > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)w.bytes);
+  w = carve_adj(P, H, dtype, batch, 0, ws);
+  DeviceTables* DF = nullptr;
+  DeviceTables* DA = nullptr;
+  if (int rc = ensure_tables(P, dtype, 0, &DF)) return rc;
+  if (int rc = ensure_tables(P, dtype, 1, &DA)) return rc;
+  const Sched& F = P.sched[dtype][0];
+  const Sched& A = P.sched[dtype][1];
+  const int64_t N = 1ll << P.n;
+  // forward with checkpoints
+  launch_prep<T>(F, *DF, DF->ctab, params, pstride, w.mtab_f, batch, st);
+  if (inputs) launch_copy<T>(w.psi, inputs, batch * N, st);
+  else launch_init_zero<T>(w.psi, P.n, batch, st);
+  if (int rc = run_forward_passes<T>(P, F, *DF, w.mtab_f, w.psi, batch, w.ckpt, st)) return rc;
+  // energy and lambda = H psi
+  if (int rc = pauli_energy_apply<T>(H, w.psi, w.lam, nullptr, batch, w.epart, true, true, st)) return rc;
+  launch_sum_slots(w.epart, (int64_t)H->nslots * 2, H->nslots, 2, w.e2, batch, st);
+  // reverse sweep
+  launch_prep<T>(A, *DA, DA->ctab, params, pstride, w.mtab_a, batch, st);
+  if (int rc = reverse_sweep<T>(P, dtype, w, w.psi, w.lam, w.ckpt, nullptr, batch, st)) return rc;
+  launch_grad_assemble(w.gsum, P.ngop, (const int32_t*)DA->p_ptr, (const int32_t*)DA->p_gops,
+                       (const double*)DA->gfac, 1.0, grad, P.nparams, batch, st);
+  // energy real parts
+  HIP_TRY(hipMemcpy2DAsync(energy, sizeof(double), w.e2, 2 * sizeof(double), sizeof(double), batch,
+                           hipMemcpyDeviceToDevice, st),
+          "energy copy");
+  HIP_TRY(hipGetLastError(), "adjoint launch");
+  return 0;
+}
+
+template <typename T>
+int vjp_impl(Program& P, int dtype, const void* params, int64_t pstride, const void* ckpts, const void* state,
+             const void* grad_state, int64_t batch, double* grad_params, void* grad_input, void* ws, int64_t ws_bytes,
+             hipStream_t st) {
+  AdjWs w = carve_adj(P, nullptr, dtype, batch, 1, nullptr);
+  if (w.bytes > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)w.bytes);
+  if (!P.ckpt_gate.empty() && !ckpts) return set_error(TCM_E_ARG, "program stores checkpoints: ckpts required");
+  w = carve_adj(P, nullptr, dtype, batch, 1, ws);
+  DeviceTables* DA = nullptr;
+  if (int rc = ensure_tables(P, dtype, 1, &DA)) return rc;
+  const Sched& A = P.sched[dtype][1];
+  launch_prep<T>(A, *DA, DA->ctab, params, pstride, w.mtab_a, batch, st);
+  if (int rc = reverse_sweep<T>(P, dtype, w, state, grad_state, ckpts, grad_input, batch, st)) return rc;
+  if (grad_params)
+    launch_grad_assemble(w.gsum, P.ngop, (const int32_t*)DA->p_ptr, (const int32_t*)DA->p_gops,
+                         (const double*)DA->gfac, 0.5, grad_params, P.nparams, batch, st);
+  HIP_TRY(hipGetLastError(), "vjp launch");
+  return 0;
+}
+
+bool valid_dtype(int d) { return d == TCM_C64 || d == TCM_C128; }
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int tcm_abi_version(void) { return TCM_ABI_VERSION; }
+
+const char* tcm_last_error(void) { return g_err.c_str(); }
+
+int tcm_program_build(const tcm_gate* gates, int32_t ngates, int32_t nqubits, int32_t nparams, const tcm_opts* opts,
+                      tcm_program** out) {
+  if (!out) return set_error(TCM_E_ARG, "out is NULL");
+  *out = nullptr;
+  Program* P = nullptr;
+  int rc = build_program(gates, ngates, nqubits, nparams, opts, &P);
+  if (rc) return rc;
+  tcm_program* tp = new tcm_program();
+  // move-construct the program into the handle
+  tp->p.n = P->n;
+  tp->p.nparams = P->nparams;
+  tp->p.opts = P->opts;
+  tp->p.gates = std::move(P->gates);
+  tp->p.nlayers = P->nlayers;
+  tp->p.ngop = P->ngop;
+  tp->p.gop_gate = std::move(P->gop_gate);
+  tp->p.p_ptr = std::move(P->p_ptr);
+  tp->p.p_gops = std::move(P->p_gops);
+  tp->p.gfac = std::move(P->gfac);
+  tp->p.ctab = std::move(P->ctab);
+  tp->p.ckpt_gate = std::move(P->ckpt_gate);
+  for (int d = 0; d < 2; ++d)
+    for (int m = 0; m < 2; ++m) tp->p.sched[d][m] = std::move(P->sched[d][m]);
+  delete P;
+  *out = tp;
+  return 0;
+}
+
+int tcm_program_free(tcm_program* prog) {
+  if (!prog) return 0;
+  int cur = -1;
+  bool have = hipGetDevice(&cur) == hipSuccess;
+  for (int d = 0; d < 2; ++d)
+    for (int m = 0; m < 2; ++m) {
+      auto& v = prog->p.sched[d][m].dev;
+      for (size_t dev = 0; dev < v.size(); ++dev)
+        if (v[dev]) {
+          (void)hipSetDevice((int)dev);
+          free_tables(v[dev]);
+          v[dev] = nullptr;
+        }
+    }
+  if (have) (void)hipSetDevice(cur);
+  (void)hipGetLastError();
+  delete prog;
+  return 0;
+}
+
+int tcm_program_get_info(const tcm_program* prog, tcm_program_info* info) {
+  if (!prog || !info) return set_error(TCM_E_ARG, "NULL argument");
+  std::memset(info, 0, sizeof *info);
+  const Program& P = prog->p;
+  info->nqubits = P.n;
+  info->nparams = P.nparams;
+  info->ngates = (int)P.gates.size();
+  info->npasses = (int)P.sched[0][0].passes.size();
+  info->nckpt = (int)P.ckpt_gate.size();
+  info->tile_bits = P.sched[0][0].k;
+  return 0;
+}
+
+int64_t tcm_program_describe(const tcm_program* prog, int32_t dtype, int32_t mode, char* buf, int64_t cap) {
+  if (!prog || !valid_dtype(dtype) || (mode != 0 && mode != 1)) return set_error(TCM_E_ARG, "bad describe args");
+  std::string s = describe_sched(prog->p, dtype, mode);
+  if (buf && cap > 0) {
+    int64_t m = std::min<int64_t>(cap - 1, (int64_t)s.size());
+    std::memcpy(buf, s.data(), m);
+    buf[m] = 0;
+  }
+  return (int64_t)s.size();
+}
+
+int tcm_forward_workspace_bytes(const tcm_program* prog, int32_t dtype, int64_t batch, int64_t* bytes) {
+  if (!prog || !bytes || !valid_dtype(dtype) || batch < 0) return set_error(TCM_E_ARG, "bad arguments");
+  *bytes = forward_ws(prog->p, dtype, batch);
+  return 0;
+}
+
+int tcm_forward(tcm_program* prog, int32_t dtype, const void* params, int64_t param_stride, void* state,
+                int64_t batch, int32_t init_zero, void* ckpts, void* workspace, int64_t ws_bytes, void* stream) {
+  if (!prog || !valid_dtype(dtype) || batch < 0 || (batch > 0 && !state))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_forward");
+  if (batch == 0) return 0;
+  if (prog->p.nparams > 0 && !params) return set_error(TCM_E_ARG, "params required");
+  if (param_stride < 0) return set_error(TCM_E_ARG, "param_stride < 0");
+  hipStream_t st = (hipStream_t)stream;
+  return dtype == TCM_C64
+             ? forward_impl<float>(prog->p, dtype, params, param_stride, state, batch, init_zero, ckpts, workspace, ws_bytes, st)
+             : forward_impl<double>(prog->p, dtype, params, param_stride, state, batch, init_zero, ckpts, workspace, ws_bytes, st);
+}
+
+int tcm_pauli_build(const int8_t* codes, const double* coeffs, int32_t nterms, int32_t nqubits, tcm_pauli** out) {
+  if (!out) return set_error(TCM_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (nqubits < 1 || nqubits > 32) return set_error(TCM_E_ARG, "nqubits=%d out of range", nqubits);
+  if (nterms < 1 || !codes || !coeffs) return set_error(TCM_E_ARG, "need at least one term");
+  std::map<uint64_t, std::vector<DPTerm>> by_flip;
+  for (int t = 0; t < nterms; ++t) {
+    if (!std::isfinite(coeffs[t])) return set_error(TCM_E_NONFINITE, "term %d: non-finite coefficient", t);
+    uint64_t flip = 0, yz = 0;
+    int ny = 0;
+    for (int q = 0; q < nqubits; ++q) {
+      int c = codes[(int64_t)t * nqubits + q];
+      if (c < 0 || c > 3) return set_error(TCM_E_ARG, "term %d: code %d at qubit %d not in 0..3", t, c, q);
+      uint64_t bit = 1ull << (nqubits - 1 - q);
+      if (c == 1) flip |= bit;
+      if (c == 2) { flip |= bit; yz |= bit; ++ny; }
+      if (c == 3) yz |= bit;
+    }
+    DPTerm tm{};
+    tm.yz = yz;
+    tm.ny = ny;
+    tm.w = coeffs[t];
+    by_flip[flip].push_back(tm);
+  }
+  tcm_pauli* H = new tcm_pauli();
+  H->n = nqubits;
+  int slot = 0;
+  for (auto& kv : by_flip) {  // std::map: flip 0 (diagonal) first
+    DPGroup g{};
+    g.flip = kv.first;
+    g.t0 = (int)H->terms.size();
+    g.nt = (int)kv.second.size();
+    g.nblk = pauli_blocks(nqubits, kv.first != 0);
+    g.slot0 = slot;
+    slot += g.nblk;
+    H->terms.insert(H->terms.end(), kv.second.begin(), kv.second.end());
+    H->groups.push_back(g);
+  }
+  H->nslots = slot;
+  *out = H;
+  return 0;
+}
+
+int tcm_pauli_free(tcm_pauli* ham) {
+  if (!ham) return 0;
+  int cur = -1;
+  bool have = hipGetDevice(&cur) == hipSuccess;
+  for (size_t d = 0; d < ham->dev.size(); ++d)
+    if (ham->dev[d].terms) {
+      (void)hipSetDevice((int)d);
+      (void)hipFree(ham->dev[d].terms);
+    }
+  if (have) (void)hipSetDevice(cur);
+  (void)hipGetLastError();
+  delete ham;
+  return 0;
+}
+
+int tcm_pauli_workspace_bytes(const tcm_pauli* ham, int32_t dtype, int64_t batch, int64_t* bytes) {
+  if (!ham || !bytes || !valid_dtype(dtype) || batch < 0) return set_error(TCM_E_ARG, "bad arguments");
+  *bytes = pauli_ws(ham, batch);
+  return 0;
+}
+
+int tcm_expect_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state, int64_t batch, double* out,
+                         void* workspace, int64_t ws_bytes, void* stream) {
+  if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_expect_pauli_sum");
+  if (batch == 0) return 0;
+  if (pauli_ws(ham, batch) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  double* epart = (double*)workspace;
+  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, state, nullptr, nullptr, batch, epart, true, false, st)
+                            : pauli_energy_apply<double>(H, state, nullptr, nullptr, batch, epart, true, false, st);
+  if (rc) return rc;
+  launch_sum_slots(epart, (int64_t)H->nslots * 2, H->nslots, 2, out, batch, st);
+  HIP_TRY(hipGetLastError(), "expect launch");
+  return 0;
+}
+
+int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state, void* out_state, int64_t batch,
+                        const double* alpha, void* stream) {
+  if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out_state)) || state == out_state)
+    return set_error(TCM_E_ARG, "bad arguments to tcm_apply_pauli_sum");
+  if (batch == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, state, out_state, alpha, batch, nullptr, false, true, st)
+                            : pauli_energy_apply<double>(H, state, out_state, alpha, batch, nullptr, false, true, st);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError(), "apply launch");
+  return 0;
+}
+
+int tcm_inner_workspace_bytes(int32_t nqubits, int64_t batch, int64_t* bytes) {
+  if (!bytes || nqubits < 1 || nqubits > 32 || batch < 0) return set_error(TCM_E_ARG, "bad arguments");
+  *bytes = align_up(batch * (int64_t)pauli_blocks(nqubits, false) * 2 * sizeof(double));
+  return 0;
+}
+
+int tcm_inner(int32_t nqubits, int32_t dtype, const void* a, const void* b, int64_t batch, double* out,
+              void* workspace, int64_t ws_bytes, void* stream) {
+  if (!valid_dtype(dtype) || nqubits < 1 || nqubits > 32 || batch < 0 || (batch > 0 && (!a || !b || !out)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_inner");
+  if (batch == 0) return 0;
+  const int nblk = pauli_blocks(nqubits, false);
+  if (align_up(batch * (int64_t)nblk * 2 * sizeof(double)) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)workspace;
+  if (dtype == TCM_C64) launch_inner<float>(a, b, nqubits, batch, nblk, part, (int64_t)nblk * 2, st);
+  else launch_inner<double>(a, b, nqubits, batch, nblk, part, (int64_t)nblk * 2, st);
+  launch_sum_slots(part, (int64_t)nblk * 2, nblk, 2, out, batch, st);
+  HIP_TRY(hipGetLastError(), "inner launch");
+  return 0;
+}
+
+int tcm_adjoint_workspace_bytes(const tcm_program* prog, const tcm_pauli* ham, int32_t dtype, int64_t batch,
+                                int32_t mode, int64_t* bytes) {
+  if (!prog || !bytes || !valid_dtype(dtype) || batch < 0 || (mode != 0 && mode != 1) || (mode == 0 && !ham))
+    return set_error(TCM_E_ARG, "bad arguments");
+  *bytes = carve_adj(prog->p, ham, dtype, batch, mode, nullptr).bytes;
+  return 0;
+}
+
+int tcm_adjoint_energy_grad(tcm_program* prog, int32_t dtype, const void* params, int64_t param_stride,
+                            const tcm_pauli* ham, const void* inputs, int64_t batch, double* energy, double* grad,
+                            void* workspace, int64_t ws_bytes, void* stream) {
+  if (!prog || !ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!energy || (!grad && prog->p.nparams > 0))))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_adjoint_energy_grad");
+  if (ham->n != prog->p.n) return set_error(TCM_E_ARG, "Hamiltonian on %d qubits, program on %d", ham->n, prog->p.n);
+  if (batch == 0) return 0;
+  if (prog->p.nparams > 0 && !params) return set_error(TCM_E_ARG, "params required");
+  hipStream_t st = (hipStream_t)stream;
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  return dtype == TCM_C64
+             ? energy_grad_impl<float>(prog->p, dtype, params, param_stride, H, inputs, batch, energy, grad, workspace, ws_bytes, st)
+             : energy_grad_impl<double>(prog->p, dtype, params, param_stride, H, inputs, batch, energy, grad, workspace, ws_bytes, st);
+}
+
+int tcm_adjoint_vjp(tcm_program* prog, int32_t dtype, const void* params, int64_t param_stride, const void* ckpts,
+                    const void* state, const void* grad_state, int64_t batch, double* grad_params, void* grad_input,
+                    void* workspace, int64_t ws_bytes, void* stream) {
+  if (!prog || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !grad_state)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_adjoint_vjp");
+  if (batch == 0) return 0;
+  if (prog->p.nparams > 0 && !params) return set_error(TCM_E_ARG, "params required");
+  hipStream_t st = (hipStream_t)stream;
+  return dtype == TCM_C64
+             ? vjp_impl<float>(prog->p, dtype, params, param_stride, ckpts, state, grad_state, batch, grad_params, grad_input, workspace, ws_bytes, st)
+             : vjp_impl<double>(prog->p, dtype, params, param_stride, ckpts, state, grad_state, batch, grad_params, grad_input, workspace, ws_bytes, st);
+}
+
+}  // extern "C"

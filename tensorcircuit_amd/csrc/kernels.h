@@ -1,0 +1,95 @@
+// Declarations shared by kernels.hip (device code + launchers) and
+// engine.cpp (the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tcm_internal.h"
+
+namespace tcm {
+
+template <typename T>
+struct V2T;
+template <>
+struct V2T<float> {
+  using type = float2;
+};
+template <>
+struct V2T<double> {
+  using type = double2;
+};
+template <typename T>
+using V2 = typename V2T<T>::type;
+
+// Pauli-sum device descriptors
+struct DPGroup {
+  uint64_t flip;   // X/Y positions (index bits)
+  int32_t t0, nt;  // terms
+  int32_t slot0;   // first energy-partial slot of this group
+  int32_t nblk;    // blocks launched (= slots used)
+};
+struct DPTerm {
+  uint64_t yz;     // Y/Z positions (index bits)
+  int32_t ny;      // number of Y factors
+  int32_t pad;
+  double w;        // weight
+};
+
+// device copies of one schedule's descriptors (owned by the program)
+struct DeviceTables {
+  void* dpass = nullptr;
+  void* dgroup = nullptr;
+  void* dop = nullptr;
+  void* dmem = nullptr;
+  void* ukbits = nullptr;
+  void* runoff = nullptr;
+  void* recipes = nullptr;
+  void* factors = nullptr;
+  void* gop_list = nullptr;
+  // program-wide (shared by both schedules of a dtype, owned by mode 0)
+  void* ctab = nullptr;    // constant matrices in T
+  void* p_ptr = nullptr;
+  void* p_gops = nullptr;
+  void* gfac = nullptr;
+};
+
+template <typename T>
+void launch_prep(const Sched& S, const DeviceTables& D, const void* ctab, const void* params, int64_t pstride,
+                 void* mtab, int64_t batch, hipStream_t st);
+template <typename T>
+void launch_init_zero(void* state, int n, int64_t batch, hipStream_t st);
+template <typename T>
+void launch_copy(void* dst, const void* src, int64_t total, hipStream_t st);
+template <typename T, bool DAG>
+void launch_plain_gate(const DOp& op, const int32_t* ukbits_dev, const void* mtab, int64_t mstride,
+                       const void* ctab, void* state, int n, int64_t batch, hipStream_t st);
+template <typename T, int R>
+void launch_fused_fwd(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
+                      void* state, int64_t batch, int n, void* ckout, hipStream_t st);
+template <typename T, int R>
+void launch_fused_adj(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
+                      const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n,
+                      double* part, int64_t pstride, hipStream_t st);
+template <typename T>
+void launch_plain_adj(const DOp& op, const void* mtab, int64_t mstride, const void* ctab, const void* psi_in,
+                      void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n, int slots, double* part,
+                      int64_t pstride, hipStream_t st);
+template <typename T>
+void launch_plain_adj_uk(const DOp& op, const int32_t* ukbits_dev, const void* ctab, void* psi, void* lam,
+                         int64_t batch, int n, hipStream_t st);
+template <typename T>
+void launch_pauli(const DPGroup& G, const DPTerm* terms_dev, int n, const void* psi, void* out, int first,
+                  const double* alpha, double* epart, int64_t estride, int64_t batch, bool energy, bool apply,
+                  hipStream_t st);
+template <typename T>
+void launch_inner(const void* a, const void* c, int n, int64_t batch, int nblk, double* part, int64_t pstride,
+                  hipStream_t st);
+void launch_sum_slots(const double* part, int64_t bstride, int64_t nslots, int ncomp, double* out, int64_t batch,
+                      hipStream_t st);
+void launch_reduce_gop(const double* part, int64_t pstride, int64_t nslots, const int32_t* gop_list, int gop_single,
+                       int ngop_pass, double* gsum, int ngop, int64_t batch, hipStream_t st);
+void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, const int32_t* p_gops,
+                          const double* gfac, double mult, double* grad, int nparams, int64_t batch,
+                          hipStream_t st);
+
+}  // namespace tcm

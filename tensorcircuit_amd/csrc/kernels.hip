@@ -1,0 +1,1286 @@
+// HIP kernels of libtcm for CDNA4 (gfx950): gate application (one kernel per
+// gate, and LDS-tiled fused passes), adjoint reverse sweeps, Pauli-sum
+// energies / H|psi>, and the fp64 reductions behind them.
+//
+// State layout: [batch][2^n] complex (float2 or double2), qubit 0 = the most
+// significant index bit (circuit.py:803-810).  A gate on qubit q acts on index
+// bit n-1-q.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <type_traits>
+
+#include "kernels.h"
+
+namespace tcm {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename T>
+__device__ __forceinline__ V2<T> mk(T a, T b) {
+  V2<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+// Im(conj(a) b), Re(conj(a) b)
+template <typename T>
+__device__ __forceinline__ T imcj(V2<T> a, V2<T> b) { return a.x * b.y - a.y * b.x; }
+template <typename T>
+__device__ __forceinline__ T recj(V2<T> a, V2<T> b) { return a.x * b.x + a.y * b.y; }
+
+__device__ __forceinline__ uint32_t swz(uint32_t t) {
+  return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu);
+}
+
+// insert a zero bit at position b
+__device__ __forceinline__ uint64_t ins0(uint64_t x, int b) {
+  uint64_t lo = x & ((1ull << b) - 1);
+  return ((x >> b) << (b + 1)) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ const T* mptr(int moff, const T* mt, const T* ct) {
+  return moff >= 0 ? mt + moff : ct + (-moff - 1);
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// block-wide sum of (a, b) in fp64; result valid in thread 0.  blockDim.x
+// must be a multiple of 64 and <= 1024.
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+  __shared__ double red[2][16];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    double sa = 0, sb = 0;
+    for (int i = 0; i < nw; ++i) {
+      sa += red[0][i];
+      sb += red[1][i];
+    }
+    a = sa;
+    b = sb;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// slot dispatch: runtime register slot -> compile-time template argument.
+// Plain switches (no lambdas capturing the register array) so that the
+// per-thread amplitude array stays in VGPRs.
+// ---------------------------------------------------------------------------
+#define TCM_SW1(R, s, ...)                                          \
+  switch (s) {                                                      \
+    case 0: { constexpr int S_ = 0; __VA_ARGS__; } break;           \
+    case 1: if constexpr (R > 1) { constexpr int S_ = 1; __VA_ARGS__; } break; \
+    case 2: if constexpr (R > 2) { constexpr int S_ = 2; __VA_ARGS__; } break; \
+    case 3: if constexpr (R > 3) { constexpr int S_ = 3; __VA_ARGS__; } break; \
+    case 4: if constexpr (R > 4) { constexpr int S_ = 4; __VA_ARGS__; } break; \
+    default: break;                                                 \
+  }
+
+#define TCM_SW2B(R, A_, s1, ...)                                                         \
+  switch (s1) {                                                                          \
+    case 0: if constexpr (A_ != 0) { constexpr int B_ = 0; __VA_ARGS__; } break;         \
+    case 1: if constexpr (R > 1 && A_ != 1) { constexpr int B_ = 1; __VA_ARGS__; } break; \
+    case 2: if constexpr (R > 2 && A_ != 2) { constexpr int B_ = 2; __VA_ARGS__; } break; \
+    case 3: if constexpr (R > 3 && A_ != 3) { constexpr int B_ = 3; __VA_ARGS__; } break; \
+    case 4: if constexpr (R > 4 && A_ != 4) { constexpr int B_ = 4; __VA_ARGS__; } break; \
+    default: break;                                                                      \
+  }
+
+#define TCM_SW2(R, s0, s1, ...)                                                           \
+  switch (s0) {                                                                           \
+    case 0: { constexpr int A_ = 0; TCM_SW2B(R, A_, s1, __VA_ARGS__) } break;             \
+    case 1: if constexpr (R > 1) { constexpr int A_ = 1; TCM_SW2B(R, A_, s1, __VA_ARGS__) } break; \
+    case 2: if constexpr (R > 2) { constexpr int A_ = 2; TCM_SW2B(R, A_, s1, __VA_ARGS__) } break; \
+    case 3: if constexpr (R > 3) { constexpr int A_ = 3; TCM_SW2B(R, A_, s1, __VA_ARGS__) } break; \
+    case 4: if constexpr (R > 4) { constexpr int A_ = 4; TCM_SW2B(R, A_, s1, __VA_ARGS__) } break; \
+    default: break;                                                                       \
+  }
+
+// ---------------------------------------------------------------------------
+// in-register gate application on v[0 .. 2^R): register index bit S <-> the
+// gate qubit.  DAG applies U^dagger.
+// ---------------------------------------------------------------------------
+template <typename T, int R, int S, bool DAG>
+__device__ __forceinline__ void ap_u1(V2<T>* v, const T* m) {
+  T ar = m[0], ai = m[1], br = m[2], bi = m[3], cr = m[4], ci = m[5], dr = m[6], di = m[7];
+  if constexpr (DAG) {
+    T tr = br, ti = bi;
+    br = cr; bi = -ci;
+    cr = tr; ci = -ti;
+    ai = -ai; di = -di;
+  }
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (1 << S)) continue;
+    V2<T> x = v[j], y = v[j | (1 << S)];
+    v[j] = mk<T>(ar * x.x - ai * x.y + br * y.x - bi * y.y, ar * x.y + ai * x.x + br * y.y + bi * y.x);
+    v[j | (1 << S)] = mk<T>(cr * x.x - ci * x.y + dr * y.x - di * y.y, cr * x.y + ci * x.x + dr * y.y + di * y.x);
+  }
+}
+
+template <typename T, int R, int S, bool DAG>
+__device__ __forceinline__ void ap_r1(V2<T>* v, const T* m) {
+  T a = m[0], b = m[1], c = m[2], d = m[3];
+  if constexpr (DAG) {
+    T t = b; b = c; c = t;
+  }
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (1 << S)) continue;
+    V2<T> x = v[j], y = v[j | (1 << S)];
+    v[j] = mk<T>(a * x.x + b * y.x, a * x.y + b * y.y);
+    v[j | (1 << S)] = mk<T>(c * x.x + d * y.x, c * x.y + d * y.y);
+  }
+}
+
+template <typename T, int R, int S, bool DAG>
+__device__ __forceinline__ void ap_d1(V2<T>* v, const T* m) {
+  T d0r = m[0], d0i = m[1], d1r = m[2], d1i = m[3];
+  if constexpr (DAG) { d0i = -d0i; d1i = -d1i; }
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    V2<T> x = v[j];
+    if (j & (1 << S)) v[j] = mk<T>(d1r * x.x - d1i * x.y, d1r * x.y + d1i * x.x);
+    else v[j] = mk<T>(d0r * x.x - d0i * x.y, d0r * x.y + d0i * x.x);
+  }
+}
+
+template <typename T, int R, int S>
+__device__ __forceinline__ void ap_x1(V2<T>* v) {
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (1 << S)) continue;
+    V2<T> t = v[j];
+    v[j] = v[j | (1 << S)];
+    v[j | (1 << S)] = t;
+  }
+}
+
+// 2-qubit: matrix index r = (bit S0) << 1 | (bit S1)
+template <typename T, int R, int S0, int S1, bool DAG>
+__device__ __forceinline__ void ap_u2(V2<T>* v, const T* m) {
+  T mr[16], mi[16];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if constexpr (DAG) {
+        mr[r * 4 + c] = m[2 * (c * 4 + r)];
+        mi[r * 4 + c] = -m[2 * (c * 4 + r) + 1];
+      } else {
+        mr[r * 4 + c] = m[2 * (r * 4 + c)];
+        mi[r * 4 + c] = m[2 * (r * 4 + c) + 1];
+      }
+    }
+  constexpr int m0 = 1 << S0, m1 = 1 << S1;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (m0 | m1)) continue;
+    const int idx[4] = {j, j | m1, j | m0, j | m0 | m1};
+    V2<T> x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      T sr = 0, si = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        sr += mr[r * 4 + c] * x[c].x - mi[r * 4 + c] * x[c].y;
+        si += mr[r * 4 + c] * x[c].y + mi[r * 4 + c] * x[c].x;
+      }
+      v[idx[r]] = mk<T>(sr, si);
+    }
+  }
+}
+
+template <typename T, int R, int S0, int S1, bool DAG>
+__device__ __forceinline__ void ap_d2(V2<T>* v, const T* m) {
+  T dr[4], di[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    dr[r] = m[2 * r];
+    di[r] = DAG ? -m[2 * r + 1] : m[2 * r + 1];
+  }
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    const int r = (((j >> S0) & 1) << 1) | ((j >> S1) & 1);
+    V2<T> x = v[j];
+    v[j] = mk<T>(dr[r] * x.x - di[r] * x.y, dr[r] * x.y + di[r] * x.x);
+  }
+}
+
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ void ap_cx(V2<T>* v) {
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (!(j & (1 << S0)) || (j & (1 << S1))) continue;
+    V2<T> t = v[j];
+    v[j] = v[j | (1 << S1)];
+    v[j | (1 << S1)] = t;
+  }
+}
+
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ void ap_cz(V2<T>* v) {
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j)
+    if ((j & (1 << S0)) && (j & (1 << S1))) v[j] = mk<T>(-v[j].x, -v[j].y);
+}
+
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ void ap_swap(V2<T>* v) {
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (!(j & (1 << S0)) || (j & (1 << S1))) continue;
+    const int jj = (j ^ (1 << S0)) | (1 << S1);
+    V2<T> t = v[j];
+    v[j] = v[jj];
+    v[jj] = t;
+  }
+}
+
+template <typename T, int R, bool DAG>
+__device__ __forceinline__ void ap_dg(V2<T>* v, const T* d) {
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    T dr = d[2 * j], di = DAG ? -d[2 * j + 1] : d[2 * j + 1];
+    V2<T> x = v[j];
+    v[j] = mk<T>(dr * x.x - di * x.y, dr * x.y + di * x.x);
+  }
+}
+
+// apply one op (kinds with a register form) on v
+template <typename T, int R, bool DAG>
+__device__ __forceinline__ void apply_op(V2<T>* v, const DOp& op, const T* mt, const T* ct) {
+  switch (op.kind) {
+    case OK_U1: {
+      const T* m = mptr<T>(op.moff, mt, ct);
+      TCM_SW1(R, op.s0, ap_u1<T, R, S_, DAG>(v, m))
+    } break;
+    case OK_R1: {
+      const T* m = mptr<T>(op.moff, mt, ct);
+      TCM_SW1(R, op.s0, ap_r1<T, R, S_, DAG>(v, m))
+    } break;
+    case OK_D1: {
+      const T* m = mptr<T>(op.moff, mt, ct);
+      TCM_SW1(R, op.s0, ap_d1<T, R, S_, DAG>(v, m))
+    } break;
+    case OK_X1:
+      TCM_SW1(R, op.s0, ap_x1<T, R, S_>(v))
+      break;
+    case OK_U2: {
+      const T* m = mptr<T>(op.moff, mt, ct);
+      TCM_SW2(R, op.s0, op.s1, ap_u2<T, R, A_, B_, DAG>(v, m))
+    } break;
+    case OK_D2: {
+      const T* m = mptr<T>(op.moff, mt, ct);
+      TCM_SW2(R, op.s0, op.s1, ap_d2<T, R, A_, B_, DAG>(v, m))
+    } break;
+    case OK_CX:
+      TCM_SW2(R, op.s0, op.s1, ap_cx<T, R, A_, B_>(v))
+      break;
+    case OK_CZ:
+      TCM_SW2(R, op.s0, op.s1, ap_cz<T, R, A_, B_>(v))
+      break;
+    case OK_SWAP:
+      TCM_SW2(R, op.s0, op.s1, ap_swap<T, R, A_, B_>(v))
+      break;
+    case OK_DG:
+      ap_dg<T, R, DAG>(v, mt + op.moff);
+      break;
+    default:
+      break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gradient kernels on registers: Im <l | G | p> summed over the thread's
+// amplitudes (p = psi after the gate, l = lambda after the gate)
+// ---------------------------------------------------------------------------
+template <typename T, int R, int S>
+__device__ __forceinline__ T g_x(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (1 << S)) continue;
+    const int jj = j | (1 << S);
+    a += imcj<T>(l[j], p[jj]) + imcj<T>(l[jj], p[j]);
+  }
+  return a;
+}
+template <typename T, int R, int S>
+__device__ __forceinline__ T g_y(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (1 << S)) continue;
+    const int jj = j | (1 << S);
+    a += recj<T>(l[jj], p[j]) - recj<T>(l[j], p[jj]);
+  }
+  return a;
+}
+template <typename T, int R, int S>
+__device__ __forceinline__ T g_z(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    T t = imcj<T>(l[j], p[j]);
+    a += (j & (1 << S)) ? -t : t;
+  }
+  return a;
+}
+template <typename T, int R, int S>
+__device__ __forceinline__ T g_p1(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j)
+    if (j & (1 << S)) a += imcj<T>(l[j], p[j]);
+  return a;
+}
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ T g_xx(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+  constexpr int m = (1 << S0) | (1 << S1);
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) a += imcj<T>(l[j], p[j ^ m]);
+  return a;
+}
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ T g_yy(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+  constexpr int m = (1 << S0) | (1 << S1);
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    T t = imcj<T>(l[j], p[j ^ m]);
+    const int par = ((j >> S0) ^ (j >> S1)) & 1;
+    a += par ? t : -t;
+  }
+  return a;
+}
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ T g_zz(const V2<T>* p, const V2<T>* l) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    T t = imcj<T>(l[j], p[j]);
+    const int par = ((j >> S0) ^ (j >> S1)) & 1;
+    a += par ? -t : t;
+  }
+  return a;
+}
+// general Hermitian generator, 2x2 (U1 layout) on slot S
+template <typename T, int R, int S>
+__device__ __forceinline__ T g_m1(const V2<T>* p, const V2<T>* l, const T* g) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (1 << S)) continue;
+    const int jj = j | (1 << S);
+    V2<T> x = p[j], y = p[jj];
+    V2<T> u = mk<T>(g[0] * x.x - g[1] * x.y + g[2] * y.x - g[3] * y.y, g[0] * x.y + g[1] * x.x + g[2] * y.y + g[3] * y.x);
+    V2<T> w = mk<T>(g[4] * x.x - g[5] * x.y + g[6] * y.x - g[7] * y.y, g[4] * x.y + g[5] * x.x + g[6] * y.y + g[7] * y.x);
+    a += imcj<T>(l[j], u) + imcj<T>(l[jj], w);
+  }
+  return a;
+}
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ T g_m2(const V2<T>* p, const V2<T>* l, const T* g) {
+  T a = 0;
+  constexpr int m0 = 1 << S0, m1 = 1 << S1;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) {
+    if (j & (m0 | m1)) continue;
+    const int idx[4] = {j, j | m1, j | m0, j | m0 | m1};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      T sr = 0, si = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const T gr = g[2 * (r * 4 + c)], gi = g[2 * (r * 4 + c) + 1];
+        sr += gr * p[idx[c]].x - gi * p[idx[c]].y;
+        si += gr * p[idx[c]].y + gi * p[idx[c]].x;
+      }
+      a += imcj<T>(l[idx[r]], mk<T>(sr, si));
+    }
+  }
+  return a;
+}
+
+template <typename T, int R>
+__device__ __forceinline__ T grad_op(const V2<T>* p, const V2<T>* l, int gen, int s0, int s1, const T* ct, int goff) {
+  T a = 0;
+  switch (gen) {
+    case GN_X: TCM_SW1(R, s0, a = (g_x<T, R, S_>(p, l))) break;
+    case GN_Y: TCM_SW1(R, s0, a = (g_y<T, R, S_>(p, l))) break;
+    case GN_Z: TCM_SW1(R, s0, a = (g_z<T, R, S_>(p, l))) break;
+    case GN_P1: TCM_SW1(R, s0, a = (g_p1<T, R, S_>(p, l))) break;
+    case GN_XX: TCM_SW2(R, s0, s1, a = (g_xx<T, R, A_, B_>(p, l))) break;
+    case GN_YY: TCM_SW2(R, s0, s1, a = (g_yy<T, R, A_, B_>(p, l))) break;
+    case GN_ZZ: TCM_SW2(R, s0, s1, a = (g_zz<T, R, A_, B_>(p, l))) break;
+    case GN_M1: TCM_SW1(R, s0, a = (g_m1<T, R, S_>(p, l, ct + goff))) break;
+    case GN_M2: TCM_SW2(R, s0, s1, a = (g_m2<T, R, A_, B_>(p, l, ct + goff))) break;
+    default: break;
+  }
+  return a;
+}
+
+// gradient of a diagonal-generator member from precomputed im[j] = Im(conj(l_j) p_j)
+template <typename T, int R, int S>
+__device__ __forceinline__ T gm_z(const T* im) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) a += ((j >> S) & 1) ? -im[j] : im[j];
+  return a;
+}
+template <typename T, int R, int S>
+__device__ __forceinline__ T gm_p1(const T* im) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j)
+    if ((j >> S) & 1) a += im[j];
+  return a;
+}
+template <typename T, int R, int S0, int S1>
+__device__ __forceinline__ T gm_zz(const T* im) {
+  T a = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << R); ++j) a += (((j >> S0) ^ (j >> S1)) & 1) ? -im[j] : im[j];
+  return a;
+}
+template <typename T, int R>
+__device__ __forceinline__ T grad_member(const T* im, int gen, int s0, int s1) {
+  T a = 0;
+  switch (gen) {
+    case GN_Z: TCM_SW1(R, s0, a = (gm_z<T, R, S_>(im))) break;
+    case GN_P1: TCM_SW1(R, s0, a = (gm_p1<T, R, S_>(im))) break;
+    case GN_ZZ: TCM_SW2(R, s0, s1, a = (gm_zz<T, R, A_, B_>(im))) break;
+    default: break;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// matrix preparation: per batch item, evaluate every recipe from params
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, const DFactor* __restrict__ fac,
+                       const T* __restrict__ params, int64_t pstride, const T* __restrict__ ct,
+                       T* __restrict__ mtab, int64_t mstride) {
+  const int64_t b = blockIdx.x;
+  T* mt = mtab + b * mstride;
+  const T* pr = params ? params + b * pstride : nullptr;
+  for (int i = threadIdx.x; i < nrec; i += blockDim.x) {
+    const DRecipe r = rec[i];
+    if (r.kind < 0) continue;
+    double a = r.param >= 0 ? r.scale * (double)pr[r.param] : r.theta;
+    T* o = mt + r.out;
+    switch (r.kind) {
+      case TCM_G_RX: {
+        double c = cos(0.5 * a), s = sin(0.5 * a);
+        o[0] = (T)c; o[1] = 0; o[2] = 0; o[3] = (T)-s; o[4] = 0; o[5] = (T)-s; o[6] = (T)c; o[7] = 0;
+      } break;
+      case TCM_G_RY: {
+        double c = cos(0.5 * a), s = sin(0.5 * a);
+        o[0] = (T)c; o[1] = (T)-s; o[2] = (T)s; o[3] = (T)c;
+      } break;
+      case TCM_G_RZ: {
+        double c = cos(0.5 * a), s = sin(0.5 * a);
+        o[0] = (T)c; o[1] = (T)-s; o[2] = (T)c; o[3] = (T)s;
+      } break;
+      case TCM_G_RZZ: {
+        double c = cos(0.5 * a), s = sin(0.5 * a);
+        o[0] = (T)c; o[1] = (T)-s; o[2] = (T)c; o[3] = (T)s;
+        o[4] = (T)c; o[5] = (T)s; o[6] = (T)c; o[7] = (T)-s;
+      } break;
+      case TCM_G_RXX:
+      case TCM_G_RYY: {
+        double c = cos(0.5 * a), s = sin(0.5 * a);
+        for (int q = 0; q < 32; ++q) o[q] = 0;
+        for (int d = 0; d < 4; ++d) o[2 * (d * 4 + d)] = (T)c;
+        if (r.kind == TCM_G_RXX) {
+          // -i s XX : anti-diagonal ones
+          for (int d = 0; d < 4; ++d) o[2 * (d * 4 + (3 - d)) + 1] = (T)-s;
+        } else {
+          // -i s YY : YY[0][3] = -1, YY[1][2] = 1, YY[2][1] = 1, YY[3][0] = -1
+          o[2 * (0 * 4 + 3) + 1] = (T)s;
+          o[2 * (1 * 4 + 2) + 1] = (T)-s;
+          o[2 * (2 * 4 + 1) + 1] = (T)-s;
+          o[2 * (3 * 4 + 0) + 1] = (T)s;
+        }
+      } break;
+      case TCM_G_PHASE: {
+        o[0] = 1; o[1] = 0; o[2] = (T)cos(a); o[3] = (T)sin(a);
+      } break;
+      case TCM_G_EXP1: {
+        // cos(a) I - i sin(a) G, G from the constant table (full matrix)
+        double c = cos(a), s = sin(a);
+        const T* g = ct + r.goff;
+        const int d = (r.layout == OK_U1 || r.layout == OK_D1) ? 2 : 4;
+        if (r.layout == OK_D1 || r.layout == OK_D2) {
+          for (int q = 0; q < d; ++q) {
+            double gr = g[2 * (q * d + q)], gi = g[2 * (q * d + q) + 1];
+            o[2 * q] = (T)(c + s * gi);
+            o[2 * q + 1] = (T)(-s * gr);
+          }
+        } else {
+          for (int rr = 0; rr < d; ++rr)
+            for (int cc = 0; cc < d; ++cc) {
+              double gr = g[2 * (rr * d + cc)], gi = g[2 * (rr * d + cc) + 1];
+              o[2 * (rr * d + cc)] = (T)((rr == cc ? c : 0.0) + s * gi);
+              o[2 * (rr * d + cc) + 1] = (T)(-s * gr);
+            }
+        }
+      } break;
+      default: break;
+    }
+  }
+  __syncthreads();
+  // fused diagonals: products of member diagonals over the group index j
+  for (int i = 0; i < nrec; ++i) {
+    const DRecipe r = rec[i];
+    if (r.kind >= 0) continue;
+    T* o = mt + r.out;
+    for (int j = threadIdx.x; j < (1 << r.R); j += blockDim.x) {
+      double pr_ = 1.0, pi_ = 0.0;
+      for (int f = 0; f < r.nfac; ++f) {
+        const DFactor F = fac[r.goff + f];
+        double er, ei;
+        if (F.layout == OK_CZ) {
+          er = (((j >> F.s0) & 1) && ((j >> F.s1) & 1)) ? -1.0 : 1.0;
+          ei = 0.0;
+        } else {
+          const T* src = F.src >= 0 ? mt + F.src : ct + (-F.src - 1);
+          int idx = F.layout == OK_D1 ? ((j >> F.s0) & 1) : ((((j >> F.s0) & 1) << 1) | ((j >> F.s1) & 1));
+          er = src[2 * idx];
+          ei = src[2 * idx + 1];
+        }
+        double nr = pr_ * er - pi_ * ei, ni = pr_ * ei + pi_ * er;
+        pr_ = nr;
+        pi_ = ni;
+      }
+      o[2 * j] = (T)pr_;
+      o[2 * j + 1] = (T)pi_;
+    }
+  }
+}
+
+// |0..0> on every state of the batch
+template <typename T>
+__global__ void k_init_zero(V2<T>* __restrict__ st, int64_t total, int64_t N) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+    st[i] = mk<T>((i % N) == 0 ? T(1) : T(0), T(0));
+}
+
+// ---------------------------------------------------------------------------
+// plain kernels: one launch per gate, grid-stride over (batch, item)
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, bool DAG>
+__global__ void __launch_bounds__(256) k_plain1(DOp op, const T* __restrict__ mtab, int64_t mstride,
+                                                const T* __restrict__ ct, V2<T>* __restrict__ st,
+                                                int64_t sstride, int n, int64_t batch) {
+  const int64_t items = 1ll << (n - 1);
+  const int64_t total = items * batch;
+  const int b0 = op.s0;
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total; w += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = w >> (n - 1);
+    const int64_t idx = w & (items - 1);
+    const uint64_t i0 = ins0(idx, b0), i1 = i0 | (1ull << b0);
+    V2<T>* s = st + b * sstride;
+    V2<T> v[2] = {s[i0], s[i1]};
+    const T* mt = mtab + b * mstride;
+    DOp o = op;
+    o.s0 = 0;
+    apply_op<T, 1, DAG>(v, o, mt, ct);
+    s[i0] = v[0];
+    s[i1] = v[1];
+  }
+}
+
+template <typename T, int KIND, bool DAG>
+__global__ void __launch_bounds__(256) k_plain2(DOp op, const T* __restrict__ mtab, int64_t mstride,
+                                                const T* __restrict__ ct, V2<T>* __restrict__ st,
+                                                int64_t sstride, int n, int64_t batch) {
+  const int64_t items = 1ll << (n - 2);
+  const int64_t total = items * batch;
+  const int ba = op.s0, bb = op.s1;  // bit of qubit 0 (matrix MSB), qubit 1
+  const int lo = ba < bb ? ba : bb, hi = ba < bb ? bb : ba;
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total; w += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = w >> (n - 2);
+    const int64_t idx = w & (items - 1);
+    const uint64_t i00 = ins0(ins0(idx, lo), hi);
+    const uint64_t ma = 1ull << ba, mb = 1ull << bb;
+    V2<T>* s = st + b * sstride;
+    // register index: bit 1 <-> qubit 0 (matrix MSB), bit 0 <-> qubit 1
+    V2<T> v[4] = {s[i00], s[i00 | mb], s[i00 | ma], s[i00 | ma | mb]};
+    const T* mt = mtab + b * mstride;
+    DOp o = op;
+    o.s0 = 1;
+    o.s1 = 0;
+    apply_op<T, 2, DAG>(v, o, mt, ct);
+    s[i00] = v[0];
+    s[i00 | mb] = v[1];
+    s[i00 | ma] = v[2];
+    s[i00 | ma | mb] = v[3];
+  }
+}
+
+// dense k-qubit gate (k = K), constant matrix; bits[0] = matrix MSB
+template <typename T, int K, bool DAG>
+__global__ void __launch_bounds__(256) k_plaink(DOp op, const int32_t* __restrict__ ukbits,
+                                                const T* __restrict__ ct, V2<T>* __restrict__ st,
+                                                int64_t sstride, int n, int64_t batch) {
+  const int64_t items = 1ll << (n - K);
+  const int64_t total = items * batch;
+  int bits[K], sorted[K];
+  for (int i = 0; i < K; ++i) sorted[i] = bits[i] = ukbits[op.mem0 + i];
+  for (int i = 0; i < K; ++i)
+    for (int j = i + 1; j < K; ++j)
+      if (sorted[j] < sorted[i]) { int t = sorted[i]; sorted[i] = sorted[j]; sorted[j] = t; }
+  const T* m = ct + (-op.moff - 1);
+  constexpr int D = 1 << K;
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total; w += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = w >> (n - K);
+    uint64_t base = w & (items - 1);
+    for (int i = 0; i < K; ++i) base = ins0(base, sorted[i]);
+    V2<T>* s = st + b * sstride;
+    V2<T> x[D];
+    uint64_t ad[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      uint64_t a = base;
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if ((c >> (K - 1 - i)) & 1) a |= 1ull << bits[i];
+      ad[c] = a;
+      x[c] = s[a];
+    }
+    for (int r = 0; r < D; ++r) {
+      T sr = 0, si = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        T mr, mi;
+        if constexpr (DAG) { mr = m[2 * (c * D + r)]; mi = -m[2 * (c * D + r) + 1]; }
+        else { mr = m[2 * (r * D + c)]; mi = m[2 * (r * D + c) + 1]; }
+        sr += mr * x[c].x - mi * x[c].y;
+        si += mr * x[c].y + mi * x[c].x;
+      }
+      s[ad[r]] = mk<T>(sr, si);
+    }
+  }
+}
+
+// plain adjoint step for a 1- or 2-qubit gate: gradient partial (if the gate
+// is trainable), then psi <- U^dag psi, lambda <- U^dag lambda.
+// grid = (slots, batch); partial[b * pstride + blockIdx.x]
+template <typename T, int NQ>
+__global__ void __launch_bounds__(256) k_plain_adj(DOp op, const T* __restrict__ mtab, int64_t mstride,
+                                                   const T* __restrict__ ct, const V2<T>* psi_in, V2<T>* psi_out,
+                                                   const V2<T>* lam_in, V2<T>* lam_out, int64_t sstride, int n,
+                                                   double* __restrict__ part, int64_t pstride) {
+  const int64_t items = 1ll << (n - NQ);
+  const int64_t b = blockIdx.y;
+  const T* mt = mtab + b * mstride;
+  const V2<T>* pi = psi_in + b * sstride;
+  const V2<T>* li = lam_in + b * sstride;
+  V2<T>* po = psi_out ? psi_out + b * sstride : nullptr;
+  V2<T>* lo_ = lam_out + b * sstride;
+  double acc = 0.0;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < items; idx += (int64_t)gridDim.x * blockDim.x) {
+    if constexpr (NQ == 1) {
+      const int b0 = op.s0;
+      const uint64_t i0 = ins0(idx, b0), i1 = i0 | (1ull << b0);
+      V2<T> p[2] = {pi[i0], pi[i1]}, l[2] = {li[i0], li[i1]};
+      if (op.gen != GN_NONE) acc += (double)grad_op<T, 1>(p, l, op.gen, 0, -1, ct, op.goff);
+      DOp o = op;
+      o.s0 = 0;
+      apply_op<T, 1, true>(p, o, mt, ct);
+      apply_op<T, 1, true>(l, o, mt, ct);
+      if (po) { po[i0] = p[0]; po[i1] = p[1]; }
+      lo_[i0] = l[0];
+      lo_[i1] = l[1];
+    } else {
+      const int ba = op.s0, bb = op.s1;
+      const int lo = ba < bb ? ba : bb, hi = ba < bb ? bb : ba;
+      const uint64_t i00 = ins0(ins0(idx, lo), hi);
+      const uint64_t ma = 1ull << ba, mb = 1ull << bb;
+      const uint64_t ad[4] = {i00, i00 | mb, i00 | ma, i00 | ma | mb};
+      V2<T> p[4], l[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) { p[c] = pi[ad[c]]; l[c] = li[ad[c]]; }
+      if (op.gen != GN_NONE) acc += (double)grad_op<T, 2>(p, l, op.gen, 1, 0, ct, op.goff);
+      DOp o = op;
+      o.s0 = 1;
+      o.s1 = 0;
+      apply_op<T, 2, true>(p, o, mt, ct);
+      apply_op<T, 2, true>(l, o, mt, ct);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (po) po[ad[c]] = p[c];
+        lo_[ad[c]] = l[c];
+      }
+    }
+  }
+  if (op.gop >= 0) {
+    double dummy = 0.0;
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) part[b * pstride + blockIdx.x] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused passes: a tile of 2^k amplitudes (k tile bits) is staged in LDS,
+// register groups of 2^R amplitudes per thread apply runs of gates, and the
+// tile is written back -- one HBM read + write for all gates of the pass.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t block_base(const DPass* P, uint64_t blk) {
+  uint64_t base = 0;
+  for (int i = 0; i < P->nblk_log; ++i)
+    if ((blk >> i) & 1) base |= 1ull << P->blk_bits[i];
+  return base;
+}
+
+template <typename T, int R, int TB>
+__global__ void __launch_bounds__(TB) k_fused_fwd(const DPass* __restrict__ passes, int pi,
+                                                    const DGroup* __restrict__ groups, const DOp* __restrict__ ops,
+                                                    const uint32_t* __restrict__ runoff, const T* __restrict__ mtab,
+                                                    int64_t mstride, const T* __restrict__ ct,
+                                                    V2<T>* __restrict__ st, int64_t sstride,
+                                                    V2<T>* __restrict__ ckout) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  V2<T>* tile = reinterpret_cast<V2<T>*>(smem);
+  const DPass* P = passes + pi;
+  const int k = P->k, L = P->L, nbl = P->nblk_log;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
+  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
+  const uint64_t base = block_base(P, blk);
+  V2<T>* s = st + b * sstride + base;
+  const T* mt = mtab + b * mstride;
+  const uint32_t* ro = runoff + P->runoff0;
+  const uint32_t lmask = (1u << L) - 1;
+#pragma unroll
+  for (int it = 0; it < (1 << R); ++it) {
+    const uint32_t a = it * nthr + tid;
+    tile[swz(a)] = s[ro[a >> L] + (a & lmask)];
+  }
+  __syncthreads();
+  for (int g = 0; g < P->ngroups; ++g) {
+    const DGroup* G = groups + P->group0 + g;
+    uint32_t tb = 0;
+    for (int i = 0; i < k - R; ++i)
+      if ((tid >> i) & 1) tb |= 1u << G->cb[i];
+    const uint32_t a0 = swz(tb);
+    V2<T> v[1 << R];
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) v[j] = tile[a0 ^ (uint32_t)G->S[j]];
+    for (int o = 0; o < G->nops; ++o) apply_op<T, R, false>(v, ops[G->op0 + o], mt, ct);
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) tile[a0 ^ (uint32_t)G->S[j]] = v[j];
+    __syncthreads();
+  }
+  V2<T>* c = ckout ? ckout + b * sstride + base : nullptr;
+#pragma unroll
+  for (int it = 0; it < (1 << R); ++it) {
+    const uint32_t a = it * nthr + tid;
+    const V2<T> x = tile[swz(a)];
+    const uint32_t off = ro[a >> L] + (a & lmask);
+    s[off] = x;
+    if (c) c[off] = x;
+  }
+}
+
+// Reverse sweep over one fused pass.  psi/lambda tiles are read from
+// psi_in/lam_in, gates are undone in reverse order, gradient partials of the
+// pass's trainable gates are reduced per tile into part[b][gloc][blk], and
+// the tiles are written to psi_out (skipped when NULL: a checkpoint replaces
+// it) and lam_out.
+template <typename T, int R, int TB>
+__global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ passes, int pi,
+                                                    const DGroup* __restrict__ groups, const DOp* __restrict__ ops,
+                                                    const DMember* __restrict__ mems,
+                                                    const uint32_t* __restrict__ runoff, const T* __restrict__ mtab,
+                                                    int64_t mstride, const T* __restrict__ ct,
+                                                    const V2<T>* psi_in, V2<T>* psi_out,
+                                                    const V2<T>* lam_in, V2<T>* lam_out, int64_t sstride,
+                                                    double* __restrict__ part, int64_t pstride) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const DPass* P = passes + pi;
+  const int k = P->k, L = P->L, nbl = P->nblk_log;
+  V2<T>* tp = reinterpret_cast<V2<T>*>(smem);
+  V2<T>* tl = tp + (1 << k);
+  double* red = reinterpret_cast<double*>(tl + (1 << k));
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
+  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
+  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
+  const uint64_t base = block_base(P, blk);
+  const T* mt = mtab + b * mstride;
+  const uint32_t* ro = runoff + P->runoff0;
+  const uint32_t lmask = (1u << L) - 1;
+  {
+    const V2<T>* sp = psi_in + b * sstride + base;
+    const V2<T>* sl = lam_in + b * sstride + base;
+#pragma unroll
+    for (int it = 0; it < (1 << R); ++it) {
+      const uint32_t a = it * nthr + tid;
+      const uint32_t off = ro[a >> L] + (a & lmask);
+      tp[swz(a)] = sp[off];
+      tl[swz(a)] = sl[off];
+    }
+  }
+  __syncthreads();
+  for (int g = P->ngroups - 1; g >= 0; --g) {
+    const DGroup* G = groups + P->group0 + g;
+    uint32_t tb = 0;
+    for (int i = 0; i < k - R; ++i)
+      if ((tid >> i) & 1) tb |= 1u << G->cb[i];
+    const uint32_t a0 = swz(tb);
+    V2<T> p[1 << R], l[1 << R];
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+      p[j] = tp[a0 ^ (uint32_t)G->S[j]];
+      l[j] = tl[a0 ^ (uint32_t)G->S[j]];
+    }
+    for (int o = G->nops - 1; o >= 0; --o) {
+      const DOp& op = ops[G->op0 + o];
+      if (op.kind == OK_DG && op.nmem > 0) {
+        T im[1 << R];
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) im[j] = imcj<T>(l[j], p[j]);
+        for (int q = 0; q < op.nmem; ++q) {
+          const DMember& M = mems[op.mem0 + q];
+          double gsum = wave_sum((double)grad_member<T, R>(im, M.gen, M.s0, M.s1));
+          if (lane == 0) red[M.gloc * nw + wv] = gsum;
+        }
+      } else if (op.gen != GN_NONE) {
+        double gsum = wave_sum((double)grad_op<T, R>(p, l, op.gen, op.s0, op.s1, ct, op.goff));
+        if (lane == 0) red[op.gloc * nw + wv] = gsum;
+      }
+      apply_op<T, R, true>(p, op, mt, ct);
+      apply_op<T, R, true>(l, op, mt, ct);
+    }
+#pragma unroll
+    for (int j = 0; j < (1 << R); ++j) {
+      tp[a0 ^ (uint32_t)G->S[j]] = p[j];
+      tl[a0 ^ (uint32_t)G->S[j]] = l[j];
+    }
+    __syncthreads();
+  }
+  {
+    V2<T>* sp = psi_out ? psi_out + b * sstride + base : nullptr;
+    V2<T>* sl = lam_out + b * sstride + base;
+#pragma unroll
+    for (int it = 0; it < (1 << R); ++it) {
+      const uint32_t a = it * nthr + tid;
+      const uint32_t off = ro[a >> L] + (a & lmask);
+      if (sp) sp[off] = tp[swz(a)];
+      sl[off] = tl[swz(a)];
+    }
+  }
+  const int ng = P->ngop;
+  for (int q = tid; q < ng; q += nthr) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[q * nw + w];
+    part[b * pstride + (int64_t)q * (1ll << nbl) + blk] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+// out[b * ncomp + c] = sum_s part[b * bstride + s * ncomp + c]
+__global__ void k_sum_slots(const double* __restrict__ part, int64_t bstride, int64_t nslots, int ncomp,
+                            double* __restrict__ out) {
+  const int64_t b = blockIdx.x;
+  double a = 0.0, c = 0.0;
+  for (int64_t s = threadIdx.x; s < nslots; s += blockDim.x) {
+    a += part[b * bstride + s * ncomp];
+    if (ncomp > 1) c += part[b * bstride + s * ncomp + 1];
+  }
+  block_sum2(a, c);
+  if (threadIdx.x == 0) {
+    out[b * ncomp] = a;
+    if (ncomp > 1) out[b * ncomp + 1] = c;
+  }
+}
+
+// gsum[b][gop] = sum over slots of part[b][gl][slot]; grid (ngop_pass, batch)
+__global__ void k_reduce_gop(const double* __restrict__ part, int64_t pstride, int64_t nslots,
+                             const int32_t* __restrict__ gop_list, int gop_single, double* __restrict__ gsum,
+                             int ngop) {
+  const int gl = blockIdx.x;
+  const int64_t b = blockIdx.y;
+  double a = 0.0, d = 0.0;
+  for (int64_t s = threadIdx.x; s < nslots; s += blockDim.x) a += part[b * pstride + gl * nslots + s];
+  block_sum2(a, d);
+  if (threadIdx.x == 0) {
+    const int gop = gop_list ? gop_list[gl] : gop_single;
+    gsum[b * ngop + gop] = a;
+  }
+}
+
+// grad[b][p] = mult * sum_{gop of p} gfac[gop] * gsum[b][gop]
+__global__ void k_grad_assemble(const double* __restrict__ gsum, int ngop, const int32_t* __restrict__ p_ptr,
+                                const int32_t* __restrict__ p_gops, const double* __restrict__ gfac, double mult,
+                                double* __restrict__ grad, int nparams, int64_t batch) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nparams * batch) return;
+  const int64_t b = i / nparams;
+  const int p = (int)(i % nparams);
+  double a = 0.0;
+  for (int q = p_ptr[p]; q < p_ptr[p + 1]; ++q) a += gfac[p_gops[q]] * gsum[b * ngop + p_gops[q]];
+  grad[i] = mult * a;
+}
+
+// ---------------------------------------------------------------------------
+// Pauli sums: one launch per flip group (terms sharing the X/Y positions).
+// (P psi)[y] = i^{ny} (-1)^{popcount((y ^ flip) & yz)} psi[y ^ flip]
+// (index rule of ps2coo_core, quantum.py:1461-1485)
+// ---------------------------------------------------------------------------
+struct PauliAcc {
+  double re, im;
+};
+
+__device__ __forceinline__ PauliAcc pauli_coef(const DPTerm* __restrict__ terms, int t0, int nt, uint64_t y) {
+  // sum_k w_k i^{ny_k} (-1)^{popcount(y & yz_k)}, y = the source index y ^ flip
+  double re = 0.0, im = 0.0;
+  for (int t = t0; t < t0 + nt; ++t) {
+    const DPTerm tm = terms[t];
+    double w = (__popcll(y & tm.yz) & 1) ? -tm.w : tm.w;
+    switch (tm.ny & 3) {
+      case 0: re += w; break;
+      case 1: im += w; break;
+      case 2: re -= w; break;
+      default: im -= w; break;
+    }
+  }
+  return {re, im};
+}
+
+template <typename T, bool ENERGY, bool APPLY>
+__global__ void __launch_bounds__(256) k_pauli(DPGroup G, const DPTerm* __restrict__ terms, int n,
+                                               const V2<T>* __restrict__ psi, int64_t sstride,
+                                               V2<T>* __restrict__ out, int first, const double* __restrict__ alpha,
+                                               double* __restrict__ epart, int64_t estride) {
+  const int64_t b = blockIdx.y;
+  const V2<T>* s = psi + b * sstride;
+  V2<T>* o = APPLY ? out + b * sstride : nullptr;
+  const double al = (APPLY && alpha) ? alpha[b] : 1.0;
+  const int64_t N = 1ll << n;
+  double er = 0.0, ei = 0.0;
+  if (G.flip) {
+    const int h = 63 - __clzll(G.flip);
+    const int64_t items = N >> 1;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < items; idx += (int64_t)gridDim.x * blockDim.x) {
+      const uint64_t y0 = ins0(idx, h), y1 = y0 ^ G.flip;
+      const V2<T> p0 = s[y0], p1 = s[y1];
+      const PauliAcc a0 = pauli_coef(terms, G.t0, G.nt, y1);  // (H psi)[y0] = a0 * psi[y1]
+      const PauliAcc a1 = pauli_coef(terms, G.t0, G.nt, y0);
+      const double h0r = a0.re * p1.x - a0.im * p1.y, h0i = a0.re * p1.y + a0.im * p1.x;
+      const double h1r = a1.re * p0.x - a1.im * p0.y, h1i = a1.re * p0.y + a1.im * p0.x;
+      if constexpr (ENERGY) {
+        er += (double)p0.x * h0r + (double)p0.y * h0i + (double)p1.x * h1r + (double)p1.y * h1i;
+        ei += (double)p0.x * h0i - (double)p0.y * h0r + (double)p1.x * h1i - (double)p1.y * h1r;
+      }
+      if constexpr (APPLY) {
+        if (first) {
+          o[y0] = mk<T>((T)(al * h0r), (T)(al * h0i));
+          o[y1] = mk<T>((T)(al * h1r), (T)(al * h1i));
+        } else {
+          V2<T> q0 = o[y0], q1 = o[y1];
+          o[y0] = mk<T>((T)(q0.x + al * h0r), (T)(q0.y + al * h0i));
+          o[y1] = mk<T>((T)(q1.x + al * h1r), (T)(q1.y + al * h1i));
+        }
+      }
+    }
+  } else {
+    for (int64_t y = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; y < N; y += (int64_t)gridDim.x * blockDim.x) {
+      const V2<T> p = s[y];
+      const PauliAcc a = pauli_coef(terms, G.t0, G.nt, (uint64_t)y);
+      const double hr = a.re * p.x - a.im * p.y, hi = a.re * p.y + a.im * p.x;
+      if constexpr (ENERGY) {
+        er += (double)p.x * hr + (double)p.y * hi;
+        ei += (double)p.x * hi - (double)p.y * hr;
+      }
+      if constexpr (APPLY) {
+        if (first) o[y] = mk<T>((T)(al * hr), (T)(al * hi));
+        else {
+          V2<T> q = o[y];
+          o[y] = mk<T>((T)(q.x + al * hr), (T)(q.y + al * hi));
+        }
+      }
+    }
+  }
+  if constexpr (ENERGY) {
+    block_sum2(er, ei);
+    if (threadIdx.x == 0) {
+      epart[b * estride + 2 * (G.slot0 + blockIdx.x)] = er;
+      epart[b * estride + 2 * (G.slot0 + blockIdx.x) + 1] = ei;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_inner(const V2<T>* __restrict__ a, const V2<T>* __restrict__ c, int n,
+                                               int64_t sstride, double* __restrict__ part, int64_t pstride) {
+  const int64_t b = blockIdx.y;
+  const int64_t N = 1ll << n;
+  double re = 0.0, im = 0.0;
+  for (int64_t y = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; y < N; y += (int64_t)gridDim.x * blockDim.x) {
+    const V2<T> x = a[b * sstride + y], z = c[b * sstride + y];
+    re += (double)x.x * z.x + (double)x.y * z.y;
+    im += (double)x.x * z.y - (double)x.y * z.x;
+  }
+  block_sum2(re, im);
+  if (threadIdx.x == 0) {
+    part[b * pstride + 2 * blockIdx.x] = re;
+    part[b * pstride + 2 * blockIdx.x + 1] = im;
+  }
+}
+
+template <typename T>
+__global__ void k_copy(V2<T>* __restrict__ dst, const V2<T>* __restrict__ src, int64_t total) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+static inline int grid_for(int64_t work, int64_t cap = 1 << 20) {
+  int64_t g = (work + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+template <typename T>
+void launch_prep(const Sched& S, const DeviceTables& D, const void* ctab, const void* params, int64_t pstride,
+                 void* mtab, int64_t batch, hipStream_t st) {
+  if (S.recipes.empty() || batch <= 0) return;
+  hipLaunchKernelGGL(k_prep<T>, dim3((unsigned)batch), dim3(256), 0, st, (const DRecipe*)D.recipes,
+                     (int)S.recipes.size(), (const DFactor*)D.factors, (const T*)params, pstride,
+                     (const T*)ctab, (T*)mtab, (int64_t)S.mtab_size);
+}
+
+template <typename T>
+void launch_init_zero(void* state, int n, int64_t batch, hipStream_t st) {
+  const int64_t total = (1ll << n) * batch;
+  hipLaunchKernelGGL(k_init_zero<T>, dim3(grid_for(total)), dim3(256), 0, st, (V2<T>*)state, total, 1ll << n);
+}
+
+template <typename T>
+void launch_copy(void* dst, const void* src, int64_t total, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(total)), dim3(256), 0, st, (V2<T>*)dst, (const V2<T>*)src, total);
+}
+
+template <typename T, bool DAG>
+void launch_plain_gate(const DOp& op, const int32_t* ukbits_dev, const void* mtab, int64_t mstride,
+                       const void* ctab, void* state, int n, int64_t batch, hipStream_t st) {
+  const T* mt = (const T*)mtab;
+  const T* ct = (const T*)ctab;
+  V2<T>* s = (V2<T>*)state;
+  const int64_t N = 1ll << n;
+  switch (op.kind) {
+    case OK_U1: case OK_R1: case OK_D1: case OK_X1:
+      hipLaunchKernelGGL((k_plain1<T, 0, DAG>), dim3(grid_for(batch * (N >> 1))), dim3(256), 0, st, op, mt, mstride,
+                         ct, s, N, n, batch);
+      break;
+    case OK_U2: case OK_D2: case OK_CX: case OK_CZ: case OK_SWAP:
+      hipLaunchKernelGGL((k_plain2<T, 0, DAG>), dim3(grid_for(batch * (N >> 2))), dim3(256), 0, st, op, mt, mstride,
+                         ct, s, N, n, batch);
+      break;
+    case OK_UK:
+      switch (op.nmem) {
+        case 3: hipLaunchKernelGGL((k_plaink<T, 3, DAG>), dim3(grid_for(batch * (N >> 3))), dim3(256), 0, st, op, ukbits_dev, ct, s, N, n, batch); break;
+        case 4: hipLaunchKernelGGL((k_plaink<T, 4, DAG>), dim3(grid_for(batch * (N >> 4))), dim3(256), 0, st, op, ukbits_dev, ct, s, N, n, batch); break;
+        case 5: hipLaunchKernelGGL((k_plaink<T, 5, DAG>), dim3(grid_for(batch * (N >> 5))), dim3(256), 0, st, op, ukbits_dev, ct, s, N, n, batch); break;
+      }
+      break;
+    default: break;
+  }
+}
+
+template <typename T>
+static size_t fused_lds_fwd(int k) { return sizeof(V2<T>) << k; }
+template <typename T>
+static size_t fused_lds_adj(int k, int ngop, int nw) {
+  return (2 * sizeof(V2<T>) << k) + sizeof(double) * (size_t)(ngop > 0 ? ngop : 1) * nw;
+}
+
+// kernels that need more than 64 KiB of dynamic LDS must opt in once
+template <typename K>
+static void ensure_lds_attr(K kern, size_t bytes) {
+  if (bytes <= 65536) return;
+  static std::mutex mu;
+  static std::map<const void*, size_t> done;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& d = done[(const void*)kern];
+  if (bytes <= d) return;
+  (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  d = bytes;
+}
+
+template <typename T, int R, int TB>
+static void launch_fused_fwd_tb(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
+                                void* state, int64_t batch, int n, void* ckout, hipStream_t st) {
+  const DPass& P = S.dpass[pi];
+  const size_t lds = fused_lds_fwd<T>(P.k);
+  ensure_lds_attr(k_fused_fwd<T, R, TB>, lds);
+  const int64_t blocks = batch << P.nblk_log;
+  hipLaunchKernelGGL((k_fused_fwd<T, R, TB>), dim3((unsigned)blocks), dim3(1u << P.tthreads_log), lds, st,
+                     (const DPass*)D.dpass, pi, (const DGroup*)D.dgroup, (const DOp*)D.dop,
+                     (const uint32_t*)D.runoff, (const T*)mtab, (int64_t)S.mtab_size, (const T*)ctab,
+                     (V2<T>*)state, (int64_t)1 << n, (V2<T>*)ckout);
+}
+
+template <typename T, int R>
+void launch_fused_fwd(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
+                      void* state, int64_t batch, int n, void* ckout, hipStream_t st) {
+  const int thr = 1 << S.dpass[pi].tthreads_log;
+  if (thr <= 256) launch_fused_fwd_tb<T, R, 256>(S, D, pi, mtab, ctab, state, batch, n, ckout, st);
+  else if (thr <= 512) launch_fused_fwd_tb<T, R, 512>(S, D, pi, mtab, ctab, state, batch, n, ckout, st);
+  else launch_fused_fwd_tb<T, R, 1024>(S, D, pi, mtab, ctab, state, batch, n, ckout, st);
+}
+
+template <typename T, int R, int TB>
+static void launch_fused_adj_tb(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
+                                const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch,
+                                int n, double* part, int64_t pstride, hipStream_t st) {
+  const DPass& P = S.dpass[pi];
+  const int nw = (1 << P.tthreads_log) >> 6;
+  const size_t lds = fused_lds_adj<T>(P.k, P.ngop, nw);
+  ensure_lds_attr(k_fused_adj<T, R, TB>, lds);
+  const int64_t blocks = batch << P.nblk_log;
+  hipLaunchKernelGGL((k_fused_adj<T, R, TB>), dim3((unsigned)blocks), dim3(1u << P.tthreads_log), lds, st,
+                     (const DPass*)D.dpass, pi, (const DGroup*)D.dgroup, (const DOp*)D.dop,
+                     (const DMember*)D.dmem, (const uint32_t*)D.runoff, (const T*)mtab, (int64_t)S.mtab_size,
+                     (const T*)ctab, (const V2<T>*)psi_in, (V2<T>*)psi_out, (const V2<T>*)lam_in, (V2<T>*)lam_out,
+                     (int64_t)1 << n, part, pstride);
+}
+
+template <typename T, int R>
+void launch_fused_adj(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
+                      const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n,
+                      double* part, int64_t pstride, hipStream_t st) {
+  const int thr = 1 << S.dpass[pi].tthreads_log;
+  if (thr <= 256)
+    launch_fused_adj_tb<T, R, 256>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st);
+  else if (thr <= 512)
+    launch_fused_adj_tb<T, R, 512>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st);
+  else
+    launch_fused_adj_tb<T, R, 1024>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st);
+}
+
+template <typename T>
+void launch_plain_adj(const DOp& op, const void* mtab, int64_t mstride, const void* ctab, const void* psi_in,
+                      void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n, int slots, double* part,
+                      int64_t pstride, hipStream_t st) {
+  dim3 grid((unsigned)slots, (unsigned)batch);
+  const int nq = (op.kind == OK_U1 || op.kind == OK_R1 || op.kind == OK_D1 || op.kind == OK_X1) ? 1 : 2;
+  if (nq == 1)
+    hipLaunchKernelGGL((k_plain_adj<T, 1>), grid, dim3(256), 0, st, op, (const T*)mtab, mstride, (const T*)ctab,
+                       (const V2<T>*)psi_in, (V2<T>*)psi_out, (const V2<T>*)lam_in, (V2<T>*)lam_out,
+                       (int64_t)1 << n, n, part, pstride);
+  else
+    hipLaunchKernelGGL((k_plain_adj<T, 2>), grid, dim3(256), 0, st, op, (const T*)mtab, mstride, (const T*)ctab,
+                       (const V2<T>*)psi_in, (V2<T>*)psi_out, (const V2<T>*)lam_in, (V2<T>*)lam_out,
+                       (int64_t)1 << n, n, part, pstride);
+}
+
+template <typename T>
+void launch_plain_adj_uk(const DOp& op, const int32_t* ukbits_dev, const void* ctab, void* psi, void* lam,
+                         int64_t batch, int n, hipStream_t st) {
+  launch_plain_gate<T, true>(op, ukbits_dev, nullptr, 0, ctab, psi, n, batch, st);
+  launch_plain_gate<T, true>(op, ukbits_dev, nullptr, 0, ctab, lam, n, batch, st);
+}
+
+template <typename T>
+void launch_pauli(const DPGroup& G, const DPTerm* terms_dev, int n, const void* psi, void* out, int first,
+                  const double* alpha, double* epart, int64_t estride, int64_t batch, bool energy, bool apply,
+                  hipStream_t st) {
+  dim3 grid((unsigned)G.nblk, (unsigned)batch);
+  const V2<T>* p = (const V2<T>*)psi;
+  V2<T>* o = (V2<T>*)out;
+  const int64_t ss = 1ll << n;
+  if (energy && apply)
+    hipLaunchKernelGGL((k_pauli<T, true, true>), grid, dim3(256), 0, st, G, terms_dev, n, p, ss, o, first, alpha, epart, estride);
+  else if (energy)
+    hipLaunchKernelGGL((k_pauli<T, true, false>), grid, dim3(256), 0, st, G, terms_dev, n, p, ss, o, first, alpha, epart, estride);
+  else
+    hipLaunchKernelGGL((k_pauli<T, false, true>), grid, dim3(256), 0, st, G, terms_dev, n, p, ss, o, first, alpha, epart, estride);
+}
+
+template <typename T>
+void launch_inner(const void* a, const void* c, int n, int64_t batch, int nblk, double* part, int64_t pstride,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_inner<T>, dim3((unsigned)nblk, (unsigned)batch), dim3(256), 0, st, (const V2<T>*)a,
+                     (const V2<T>*)c, n, (int64_t)1 << n, part, pstride);
+}
+
+void launch_sum_slots(const double* part, int64_t bstride, int64_t nslots, int ncomp, double* out, int64_t batch,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_slots, dim3((unsigned)batch), dim3(256), 0, st, part, bstride, nslots, ncomp, out);
+}
+
+void launch_reduce_gop(const double* part, int64_t pstride, int64_t nslots, const int32_t* gop_list, int gop_single,
+                       int ngop_pass, double* gsum, int ngop, int64_t batch, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_gop, dim3((unsigned)ngop_pass, (unsigned)batch), dim3(256), 0, st, part, pstride,
+                     nslots, gop_list, gop_single, gsum, ngop);
+}
+
+void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, const int32_t* p_gops,
+                          const double* gfac, double mult, double* grad, int nparams, int64_t batch,
+                          hipStream_t st) {
+  const int64_t total = (int64_t)nparams * batch;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_grad_assemble, dim3(grid_for(total, 1 << 24)), dim3(256), 0, st, gsum, ngop, p_ptr, p_gops,
+                     gfac, mult, grad, nparams, batch);
+}
+
+// explicit instantiations used by engine.cpp
+#define TCM_INST(T)                                                                                            \
+  template void launch_prep<T>(const Sched&, const DeviceTables&, const void*, const void*, int64_t, void*,   \
+                               int64_t, hipStream_t);                                                          \
+  template void launch_init_zero<T>(void*, int, int64_t, hipStream_t);                                         \
+  template void launch_copy<T>(void*, const void*, int64_t, hipStream_t);                                      \
+  template void launch_plain_gate<T, false>(const DOp&, const int32_t*, const void*, int64_t, const void*,    \
+                                            void*, int, int64_t, hipStream_t);                                 \
+  template void launch_plain_gate<T, true>(const DOp&, const int32_t*, const void*, int64_t, const void*,     \
+                                           void*, int, int64_t, hipStream_t);                                  \
+  template void launch_fused_fwd<T, 4>(const Sched&, const DeviceTables&, int, const void*, const void*,      \
+                                       void*, int64_t, int, void*, hipStream_t);                               \
+  template void launch_fused_adj<T, 4>(const Sched&, const DeviceTables&, int, const void*, const void*,      \
+                                       const void*, void*, const void*, void*, int64_t, int, double*, int64_t, \
+                                       hipStream_t);                                                           \
+  template void launch_plain_adj<T>(const DOp&, const void*, int64_t, const void*, const void*, void*,        \
+                                    const void*, void*, int64_t, int, int, double*, int64_t, hipStream_t);     \
+  template void launch_plain_adj_uk<T>(const DOp&, const int32_t*, const void*, void*, void*, int64_t, int,   \
+                                       hipStream_t);                                                           \
+  template void launch_pauli<T>(const DPGroup&, const DPTerm*, int, const void*, void*, int, const double*,   \
+                                double*, int64_t, int64_t, bool, bool, hipStream_t);                           \
+  template void launch_inner<T>(const void*, const void*, int, int64_t, int, double*, int64_t, hipStream_t);
+
+TCM_INST(float)
+TCM_INST(double)
+
+}  // namespace tcm

@@ -1,0 +1,484 @@
+"""Python side of the engine: owners of the C-ABI handles (gate programs,
+Pauli sums), device workspaces, and the torch.autograd.Function bridges.
+
+The reference's gradients are reverse-mode AD of the framework through the
+tensordot network (backends/pytorch_backend.py:497-544) and its batching is
+torch.vmap (backends/pytorch_backend.py:605-617).  Here the state is produced
+by one HIP launch sequence and differentiated by the adjoint reverse sweep;
+the autograd.Functions below carry explicit vmap rules so that
+``torch.func.grad_and_value`` / ``torch.vmap`` (what the backend's grad /
+vmap / vvag call) compose with the native calls, the pattern of the
+reference's torch bridge (interfaces/torch.py:68-111).
+"""
+from __future__ import annotations
+
+import collections
+import ctypes as C
+import json
+import threading
+from typing import Any, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+# ---------------------------------------------------------------------------
+# device / dtype helpers
+# ---------------------------------------------------------------------------
+
+_CDT = {"complex64": torch.complex64, "complex128": torch.complex128}
+_RDT = {"complex64": torch.float32, "complex128": torch.float64}
+_CODE = {"complex64": _lib.TCM_C64, "complex128": _lib.TCM_C128}
+
+
+def device() -> torch.device:
+    """The HIP device every state lives on; raises without one (no CPU path)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("tensorcircuit_amd computes on a HIP device and none is available")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _workspace(nbytes: int, dev: torch.device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
+
+
+def _i64() -> C.c_int64:
+    return C.c_int64(0)
+
+
+# ---------------------------------------------------------------------------
+# gate programs
+# ---------------------------------------------------------------------------
+
+
+class GateSpec(collections.namedtuple("GateSpec", "kind qubits param theta scale mat")):
+    """One gate for the C ABI: tcm_gate_kind, qubit tuple, parameter slot
+    (-1 = fixed), fixed angle, angle scale, constant / generator matrix."""
+
+    def key(self) -> Tuple:
+        m = None if self.mat is None else np.ascontiguousarray(self.mat, dtype=np.complex128).tobytes()
+        return (self.kind, tuple(self.qubits), self.param >= 0, float(self.theta) if self.param < 0 else 0.0,
+                float(self.scale), m)
+
+
+def barrier() -> GateSpec:
+    return GateSpec(_lib.G_BARRIER, (), -1, 0.0, 1.0, None)
+
+
+class Program:
+    """A built gate program (tcm_program_build).  ``fuse`` selects LDS-tiled
+    fused passes (1) or one kernel per gate (0, the plain path)."""
+
+    def __init__(self, n: int, specs: Sequence[GateSpec], nparams: int, fuse: int = 1, tile_bits: int = 0,
+                 adj_tile_bits: int = 0, ckpt_every: int = 0, reg_slots: int = 0):
+        self.n = int(n)
+        self.nparams = int(nparams)
+        self.specs = list(specs)
+        arr = (_lib.TcmGate * max(len(self.specs), 1))()
+        keep = []
+        for i, g in enumerate(self.specs):
+            arr[i].kind = g.kind
+            arr[i].nq = len(g.qubits)
+            for j, q in enumerate(g.qubits):
+                arr[i].q[j] = int(q)
+            arr[i].param = int(g.param)
+            arr[i].theta = float(g.theta)
+            arr[i].scale = float(g.scale)
+            if g.mat is not None:
+                m = np.ascontiguousarray(np.asarray(g.mat, dtype=np.complex128)).view(np.float64).copy()
+                keep.append(m)
+                arr[i].mat = m.ctypes.data_as(C.POINTER(C.c_double))
+        opts = _lib.TcmOpts()
+        opts.fuse, opts.tile_bits, opts.adj_tile_bits = fuse, tile_bits, adj_tile_bits
+        opts.ckpt_every, opts.reg_slots = ckpt_every, reg_slots
+        h = C.c_void_p()
+        check(lib.tcm_program_build(arr, len(self.specs), self.n, self.nparams, C.byref(opts), C.byref(h)),
+              "tcm_program_build")
+        self.handle = h
+        info = _lib.TcmProgramInfo()
+        check(lib.tcm_program_get_info(self.handle, C.byref(info)), "tcm_program_get_info")
+        self.nckpt = info.nckpt
+        self.npasses = info.npasses
+        self.ngates = info.ngates
+        self.tile_bits = info.tile_bits
+        self.fuse = fuse
+
+    def __del__(self, _free=lib.tcm_program_free):
+        h = getattr(self, "handle", None)
+        if h:
+            _free(h)
+            self.handle = None
+
+    def describe(self, dtype: str = "complex64", mode: int = 0) -> Dict:
+        n = lib.tcm_program_describe(self.handle, _CODE[dtype], mode, None, 0)
+        buf = C.create_string_buffer(int(n) + 1)
+        lib.tcm_program_describe(self.handle, _CODE[dtype], mode, buf, int(n) + 1)
+        return json.loads(buf.value.decode())
+
+    # --- forward -------------------------------------------------------------
+    def forward(self, params: Optional[torch.Tensor], batch: int, dtype: str = "complex64",
+                inputs: Optional[torch.Tensor] = None, want_ckpts: bool = False,
+                param_stride: Optional[int] = None,
+                out: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+        dev = device()
+        N = 1 << self.n
+        code = _CODE[dtype]
+        if out is None:
+            if inputs is not None:
+                state = inputs.to(device=dev, dtype=_CDT[dtype]).reshape(batch, N).clone()
+            else:
+                state = torch.empty(batch, N, dtype=_CDT[dtype], device=dev)
+        else:
+            state = out
+            if inputs is not None:
+                state.copy_(inputs.reshape(batch, N))
+        p = None
+        if self.nparams > 0:
+            p = params.to(device=dev, dtype=_RDT[dtype]).contiguous()
+        pstride = self.nparams if param_stride is None else param_stride
+        ck = torch.empty((self.nckpt if want_ckpts else 0, batch, N), dtype=_CDT[dtype], device=dev)
+        nb = _i64()
+        check(lib.tcm_forward_workspace_bytes(self.handle, code, batch, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, dev)
+        check(lib.tcm_forward(self.handle, code, _ptr(p), pstride, state.data_ptr(), batch,
+                              1 if inputs is None else 0, _ptr(ck) if want_ckpts and self.nckpt else None,
+                              ws.data_ptr(), ws.numel(), _stream()), "tcm_forward")
+        return state, ck
+
+    # --- vector-Jacobian product ---------------------------------------------
+    def vjp(self, params: Optional[torch.Tensor], ckpts: Optional[torch.Tensor], state: torch.Tensor,
+            grad_state: torch.Tensor, batch: int, dtype: str, want_input: bool,
+            param_stride: Optional[int] = None) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
+        dev = device()
+        code = _CODE[dtype]
+        p = None if self.nparams == 0 else params.to(device=dev, dtype=_RDT[dtype]).contiguous()
+        pstride = self.nparams if param_stride is None else param_stride
+        st = state.to(_CDT[dtype]).contiguous()
+        gs = grad_state.to(_CDT[dtype]).contiguous()
+        ck = None if (ckpts is None or self.nckpt == 0) else ckpts.to(_CDT[dtype]).contiguous()
+        gp = torch.empty(batch, max(self.nparams, 1), dtype=torch.float64, device=dev)
+        gi = torch.empty_like(st) if want_input else None
+        nb = _i64()
+        check(lib.tcm_adjoint_workspace_bytes(self.handle, None, code, batch, 1, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, dev)
+        check(lib.tcm_adjoint_vjp(self.handle, code, _ptr(p), pstride, _ptr(ck), st.data_ptr(), gs.data_ptr(),
+                                  batch, gp.data_ptr() if self.nparams else None, _ptr(gi), ws.data_ptr(),
+                                  ws.numel(), _stream()), "tcm_adjoint_vjp")
+        return gp[:, : self.nparams], gi
+
+    # --- fused energy + gradient ---------------------------------------------
+    def energy_grad(self, params: torch.Tensor, ham: "PauliSum", batch: int, dtype: str = "complex64",
+                    inputs: Optional[torch.Tensor] = None,
+                    param_stride: Optional[int] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+        dev = device()
+        code = _CODE[dtype]
+        p = params.to(device=dev, dtype=_RDT[dtype]).contiguous() if self.nparams else None
+        pstride = self.nparams if param_stride is None else param_stride
+        inp = None if inputs is None else inputs.to(device=dev, dtype=_CDT[dtype]).contiguous()
+        e = torch.empty(batch, dtype=torch.float64, device=dev)
+        g = torch.empty(batch, max(self.nparams, 1), dtype=torch.float64, device=dev)
+        nb = _i64()
+        check(lib.tcm_adjoint_workspace_bytes(self.handle, ham.handle, code, batch, 0, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, dev)
+        check(lib.tcm_adjoint_energy_grad(self.handle, code, _ptr(p), pstride, ham.handle, _ptr(inp), batch,
+                                          e.data_ptr(), g.data_ptr(), ws.data_ptr(), ws.numel(), _stream()),
+              "tcm_adjoint_energy_grad")
+        return e, g[:, : self.nparams]
+
+    def adjoint_workspace_bytes(self, ham: Optional["PauliSum"], batch: int, dtype: str, mode: int) -> int:
+        nb = _i64()
+        check(lib.tcm_adjoint_workspace_bytes(self.handle, None if ham is None else ham.handle, _CODE[dtype],
+                                              batch, mode, C.byref(nb)), "workspace")
+        return nb.value
+
+
+_PROG_CACHE: "collections.OrderedDict[Tuple, Program]" = collections.OrderedDict()
+_PROG_LOCK = threading.Lock()
+_PROG_CACHE_MAX = 128
+
+
+def get_program(n: int, specs: Sequence[GateSpec], nparams: int, **opts: Any) -> Program:
+    """Build-once cache keyed by structure (gate kinds, qubits, constant
+    matrices, parameter slots) and options -- the reference rebuilds its
+    tensor network on every call (basecircuit.py:120-245)."""
+    key = (n, nparams, tuple(s.key() + (s.param,) for s in specs), tuple(sorted(opts.items())))
+    with _PROG_LOCK:
+        p = _PROG_CACHE.get(key)
+        if p is not None:
+            _PROG_CACHE.move_to_end(key)
+            return p
+    p = Program(n, specs, nparams, **opts)
+    with _PROG_LOCK:
+        _PROG_CACHE[key] = p
+        while len(_PROG_CACHE) > _PROG_CACHE_MAX:
+            _PROG_CACHE.popitem(last=False)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Pauli sums
+# ---------------------------------------------------------------------------
+
+
+class PauliSum:
+    """Sum_k w_k P_k with P_k given by codes 0=I,1=X,2=Y,3=Z per qubit
+    (the ``ps`` structures of quantum.py:1025-1068 / any_measurements)."""
+
+    def __init__(self, n: int, codes: Any, coeffs: Any):
+        codes = np.ascontiguousarray(np.asarray(codes, dtype=np.int8).reshape(-1, n))
+        coeffs = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64).reshape(-1))
+        if codes.shape[0] != coeffs.shape[0]:
+            raise ValueError("codes and coeffs disagree on the number of terms")
+        self.n = n
+        self.codes = codes
+        self.coeffs = coeffs
+        h = C.c_void_p()
+        check(lib.tcm_pauli_build(codes.ctypes.data_as(C.POINTER(C.c_int8)),
+                                  coeffs.ctypes.data_as(C.POINTER(C.c_double)), codes.shape[0], n, C.byref(h)),
+              "tcm_pauli_build")
+        self.handle = h
+
+    def __del__(self, _free=lib.tcm_pauli_free):
+        h = getattr(self, "handle", None)
+        if h:
+            _free(h)
+            self.handle = None
+
+    def expect(self, state: torch.Tensor) -> torch.Tensor:
+        """complex128 [batch] of <psi|H|psi> (fp64 accumulation)."""
+        dev = device()
+        code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
+        s = state.reshape(-1, 1 << self.n).contiguous()
+        b = s.shape[0]
+        out = torch.empty(b, 2, dtype=torch.float64, device=dev)
+        nb = _i64()
+        check(lib.tcm_pauli_workspace_bytes(self.handle, code, b, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, dev)
+        check(lib.tcm_expect_pauli_sum(self.handle, code, s.data_ptr(), b, out.data_ptr(), ws.data_ptr(),
+                                       ws.numel(), _stream()), "tcm_expect_pauli_sum")
+        return torch.view_as_complex(out)
+
+    def apply(self, state: torch.Tensor, alpha: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """alpha_b * H psi_b as a new state tensor."""
+        code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
+        s = state.reshape(-1, 1 << self.n).contiguous()
+        out = torch.empty_like(s)
+        a = None if alpha is None else alpha.to(torch.float64).contiguous()
+        check(lib.tcm_apply_pauli_sum(self.handle, code, s.data_ptr(), out.data_ptr(), s.shape[0], _ptr(a),
+                                      _stream()), "tcm_apply_pauli_sum")
+        return out
+
+
+_PAULI_CACHE: "collections.OrderedDict[Tuple, PauliSum]" = collections.OrderedDict()
+
+
+def get_pauli(n: int, codes: Any, coeffs: Any) -> PauliSum:
+    codes = np.asarray(codes, dtype=np.int8).reshape(-1, n)
+    coeffs = np.asarray(coeffs, dtype=np.float64).reshape(-1)
+    key = (n, codes.tobytes(), coeffs.tobytes())
+    with _PROG_LOCK:
+        h = _PAULI_CACHE.get(key)
+        if h is not None:
+            _PAULI_CACHE.move_to_end(key)
+            return h
+    h = PauliSum(n, codes, coeffs)
+    with _PROG_LOCK:
+        _PAULI_CACHE[key] = h
+        while len(_PAULI_CACHE) > 256:
+            _PAULI_CACHE.popitem(last=False)
+    return h
+
+
+def inner(a: torch.Tensor, b: torch.Tensor, n: int) -> torch.Tensor:
+    """complex128 [batch] of <a|b>."""
+    dev = device()
+    code = _CODE["complex64" if a.dtype == torch.complex64 else "complex128"]
+    a2 = a.reshape(-1, 1 << n).contiguous()
+    b2 = b.to(a.dtype).reshape(-1, 1 << n).contiguous()
+    bt = a2.shape[0]
+    out = torch.empty(bt, 2, dtype=torch.float64, device=dev)
+    nb = _i64()
+    check(lib.tcm_inner_workspace_bytes(n, bt, C.byref(nb)), "workspace")
+    ws = _workspace(nb.value, dev)
+    check(lib.tcm_inner(n, code, a2.data_ptr(), b2.data_ptr(), bt, out.data_ptr(), ws.data_ptr(), ws.numel(),
+                        _stream()), "tcm_inner")
+    return torch.view_as_complex(out)
+
+
+# ---------------------------------------------------------------------------
+# autograd bridges with vmap rules
+# ---------------------------------------------------------------------------
+
+
+def _to_batch(x: Optional[torch.Tensor], dim: Optional[int], B: int, tail: Tuple[int, ...]) -> Optional[torch.Tensor]:
+    """Physical batched view [B, *tail] of a vmap argument."""
+    if x is None:
+        return None
+    if dim is None:
+        return x.reshape((1,) + tail).expand((B,) + tail).contiguous()
+    return x.movedim(dim, 0).reshape((B,) + tail).contiguous()
+
+
+class StateFn(torch.autograd.Function):
+    """params [P] (and optional input state [N]) -> (psi [N], checkpoints).
+    Backward = adjoint VJP through StateVJPFn (which has its own vmap rule)."""
+
+    @staticmethod
+    def forward(params, inputs, prog, dtype):
+        N = 1 << prog.n
+        st, ck = prog.forward(params.reshape(1, -1) if prog.nparams else None, 1, dtype,
+                              inputs=None if inputs is None else inputs.reshape(1, N), want_ckpts=prog.nckpt > 0)
+        return st.reshape(N), ck.reshape(ck.shape[0], N)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        params, inp, prog, dtype = inputs
+        st, ck = output
+        ctx.save_for_backward(params, st, ck)
+        ctx.prog, ctx.dtype, ctx.has_input = prog, dtype, inp is not None
+        ctx.mark_non_differentiable(ck)
+
+    @staticmethod
+    def backward(ctx, g_state, g_ck):
+        params, st, ck = ctx.saved_tensors
+        want_input = bool(ctx.has_input and ctx.needs_input_grad[1])
+        gp, gi = StateVJPFn.apply(params, ck, st, g_state, ctx.prog, ctx.dtype, want_input)
+        return (gp if ctx.needs_input_grad[0] else None), (gi if want_input else None), None, None
+
+    @staticmethod
+    def vmap(info, in_dims, params, inputs, prog, dtype):
+        B = info.batch_size
+        N = 1 << prog.n
+        P = params.shape[-1] if params.dim() else 0
+        p = _to_batch(params, in_dims[0], B, (P,)) if prog.nparams else None
+        inp = _to_batch(inputs, in_dims[1], B, (N,)) if inputs is not None else None
+        st, ck = prog.forward(p, B, dtype, inputs=inp, want_ckpts=prog.nckpt > 0)
+        return (st, ck), (0, 1)
+
+
+class StateVJPFn(torch.autograd.Function):
+    """(params, ckpts, psi, dL/dpsi) -> (dL/dparams, dL/dinput) by the reverse sweep."""
+
+    @staticmethod
+    def forward(params, ck, st, g_state, prog, dtype, want_input):
+        N = 1 << prog.n
+        gp, gi = prog.vjp(params.reshape(1, -1) if prog.nparams else None, ck.reshape(ck.shape[0], 1, N),
+                          st.reshape(1, N), g_state.reshape(1, N), 1, dtype, want_input)
+        gp = gp.reshape(params.shape).to(params.dtype)
+        gi = gi.reshape(-1) if want_input else torch.zeros(0, dtype=st.dtype, device=st.device)
+        return gp, gi
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, *grads):
+        raise NotImplementedError("higher-order derivatives through the adjoint sweep are not supported")
+
+    @staticmethod
+    def vmap(info, in_dims, params, ck, st, g_state, prog, dtype, want_input):
+        B = info.batch_size
+        N = 1 << prog.n
+        P = params.shape[-1] if params.dim() else 0
+        p = _to_batch(params, in_dims[0], B, (P,)) if prog.nparams else None
+        C_ = ck.shape[0] if in_dims[1] != 0 else ck.shape[1]
+        if in_dims[1] is None:
+            ckb = ck.reshape(C_, 1, N).expand(C_, B, N).contiguous()
+        else:
+            ckb = ck.movedim(in_dims[1], 1).reshape(C_, B, N).contiguous()
+        s = _to_batch(st, in_dims[2], B, (N,))
+        g = _to_batch(g_state, in_dims[3], B, (N,))
+        gp, gi = prog.vjp(p, ckb, s, g, B, dtype, want_input)
+        gp = gp.to(params.dtype).reshape((B,) + tuple(params.shape[-1:]) if params.dim() else (B,))
+        if not want_input:
+            gi = torch.zeros(B, 0, dtype=s.dtype, device=s.device)
+        return (gp, gi), (0, 0)
+
+
+class PauliExpFn(torch.autograd.Function):
+    """psi [N] -> <psi|H|psi> (complex, the state dtype), fp64-accumulated."""
+
+    @staticmethod
+    def forward(st, ham):
+        return ham.expect(st.reshape(1, -1)).to(st.dtype).reshape(())
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, ham = inputs
+        ctx.save_for_backward(st)
+        ctx.ham = ham
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        return PauliExpVJPFn.apply(st, g, ctx.ham), None
+
+    @staticmethod
+    def vmap(info, in_dims, st, ham):
+        B = info.batch_size
+        s = _to_batch(st, in_dims[0], B, (1 << ham.n,))
+        return ham.expect(s).to(st.dtype), 0
+
+
+class PauliExpVJPFn(torch.autograd.Function):
+    """dL/dpsi = 2 Re(g) H psi for L depending on E = <psi|H|psi> (H Hermitian)."""
+
+    @staticmethod
+    def forward(st, g, ham):
+        alpha = (2.0 * torch.real(g)).to(torch.float64).reshape(1).to(st.device)
+        return ham.apply(st.reshape(1, -1), alpha).reshape(st.shape)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, *grads):
+        raise NotImplementedError("higher-order derivatives of expectations are not supported")
+
+    @staticmethod
+    def vmap(info, in_dims, st, g, ham):
+        B = info.batch_size
+        N = 1 << ham.n
+        s = _to_batch(st, in_dims[0], B, (N,))
+        gg = _to_batch(g, in_dims[1], B, ())
+        alpha = (2.0 * torch.real(gg)).to(torch.float64).reshape(B).to(s.device)
+        return ham.apply(s, alpha), 0
+
+
+class OpExpFn(torch.autograd.Function):
+    """psi -> <psi|O|psi> for a general operator O given as two programs
+    (O and O^dagger); dL/dpsi = conj(g) O psi + g O^dagger psi."""
+
+    @staticmethod
+    def forward(st, prog, prog_dag, dtype):
+        phi, _ = prog.forward(None, 1, dtype, inputs=st.reshape(1, -1))
+        return inner(st.reshape(1, -1), phi, prog.n).to(st.dtype).reshape(())
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, prog, prog_dag, dtype = inputs
+        ctx.save_for_backward(st)
+        ctx.progs = (prog, prog_dag, dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        prog, prog_dag, dtype = ctx.progs
+        with torch.no_grad():
+            a, _ = prog.forward(None, 1, dtype, inputs=st.reshape(1, -1))
+            b, _ = prog_dag.forward(None, 1, dtype, inputs=st.reshape(1, -1))
+            gg = g.to(st.dtype)
+            out = (torch.conj(gg) * a + gg * b).reshape(st.shape)
+        return out, None, None, None

@@ -1,0 +1,57 @@
+"""Measurement shortcuts (the reference's templates/measurements.py) on the
+fused fp64 Pauli-sum kernel: all terms of a Hamiltonian in one call instead
+of one contraction per term."""
+from __future__ import annotations
+
+from typing import Any, Sequence
+
+import numpy as np
+import torch
+
+from .. import engine
+
+
+def _codes_of(structures: Any, n: int) -> np.ndarray:
+    s = structures.detach().cpu().numpy() if isinstance(structures, torch.Tensor) else np.asarray(structures)
+    s = np.asarray(s)
+    if s.ndim == 3:  # onehot [nterms, n, 4]
+        s = np.argmax(s, axis=-1)
+    return s.reshape(-1, n).astype(np.int8)
+
+
+def pauli_sum_expectation(c: Any, structures: Any, weights: Any = None, reuse: bool = True) -> Any:
+    """sum_k w_k <P_k> as a complex scalar, one fused fp64 reduction; the
+    gradient flows through one H|psi> application."""
+    n = c._nstate
+    codes = _codes_of(structures, n)
+    w = np.ones(codes.shape[0]) if weights is None else np.asarray(
+        weights.detach().cpu().numpy() if isinstance(weights, torch.Tensor) else weights, dtype=np.float64)
+    return c._pauli_expectation(codes, w.reshape(-1), reuse)
+
+
+def any_measurements(c: Any, structures: Any, onehot: bool = False, reuse: bool = True) -> Any:
+    """Real expectation of each Pauli structure (templates/measurements.py:18-83);
+    a single structure returns a scalar."""
+    n = c._nstate
+    codes = _codes_of(structures, n)
+    vals = [torch.real(c._pauli_expectation(codes[k:k + 1], np.ones(1), reuse)) for k in range(codes.shape[0])]
+    if np.asarray(structures).ndim == (2 if onehot else 1):
+        return vals[0]
+    return torch.stack(vals)
+
+
+def parameterized_local_measurements(c: Any, structures: Any, onehot: bool = False) -> Any:
+    """<P_i> of the single-qubit Pauli on every qubit (templates/measurements.py:86-120)."""
+    n = c._nstate
+    s = _codes_of(structures, n).reshape(-1)
+    vals = []
+    for i in range(n):
+        codes = np.zeros((1, n), dtype=np.int8)
+        codes[0, i] = s[i]
+        vals.append(torch.real(c._pauli_expectation(codes, np.ones(1))))
+    return torch.stack(vals)
+
+
+def operator_expectation(c: Any, h: "engine.PauliSum") -> Any:
+    """<psi|H|psi> for a PauliSum (templates/measurements.py:156-174)."""
+    return engine.PauliExpFn.apply(c._state(), h)

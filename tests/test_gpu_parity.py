@@ -1,0 +1,473 @@
+"""GPU parity of the HIP path against the complex128 CPU oracle.
+
+Tolerances (from the target): complex64 results within 1e-5 (relative to
+the state norm / energy scale), complex128 within 1e-12 (1e-10 for values
+accumulated over hundreds of gates), gradients max|d| <= 1e-5 (c64).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import statevector as sv
+
+pytestmark = pytest.mark.gpu
+
+C64_TOL = 1e-5
+C128_TOL = 1e-10
+
+
+@pytest.fixture(autouse=True)
+def _reset_dtype():
+    import tensorcircuit_amd as tc
+
+    tc.set_dtype("complex64")
+    yield
+    tc.set_dtype("complex64")
+
+
+def np_of(t):
+    return t.detach().cpu().numpy()
+
+
+# ------------------------------------------------------------------ KATs ---
+
+def test_wavefunction_index_order(gpu):
+    """tests/test_circuit.py:21-43."""
+    import tensorcircuit_amd as tc
+
+    m = np.arange(16).reshape(2, 2, 2, 2).astype(np.complex64)
+    qc = tc.Circuit(2)
+    qc.unitary(0, 1, unitary=tc.gates.Gate(m))
+    assert np.real(np_of(qc.wavefunction())[2]) == 8
+    qc = tc.Circuit(2)
+    qc.unitary(1, 0, unitary=tc.gates.Gate(m))
+    assert np.real(np_of(qc.wavefunction())[2]) == 4
+    qc = tc.Circuit(2)
+    qc.unitary(0, unitary=tc.gates.Gate(np.arange(4).reshape(2, 2).astype(np.complex64)))
+    assert np.real(np_of(qc.wavefunction())[2]) == 2
+
+
+def test_basics_amplitude(gpu):
+    """tests/test_circuit.py:46-52."""
+    import tensorcircuit_amd as tc
+
+    c = tc.Circuit(2)
+    c.x(0)
+    np.testing.assert_allclose(np_of(c.amplitude("10")), 1.0)
+    c.CNOT(0, 1)
+    np.testing.assert_allclose(np_of(c.amplitude("11")), 1.0)
+
+
+def test_gates_in_circuit_and_controls(gpu):
+    """tests/test_circuit.py:64-84."""
+    import tensorcircuit_amd as tc
+
+    c = tc.Circuit(2, inputs=np.eye(2**2))
+    c.iswap(0, 1)
+    np.testing.assert_allclose(np_of(c.state()).reshape(4, 4), sv.iswap(), atol=1e-5)
+    c = tc.Circuit(2)
+    c.x(1)
+    c.crx(1, 0, theta=0.3)
+    np.testing.assert_allclose(np_of(c.expectation([tc.gates._z_matrix, 0])), 0.95533645, atol=1e-5)
+    c = tc.Circuit(1)
+    c.X(0)
+    c.SD(0)
+    np.testing.assert_allclose(np_of(c.state()), np.array([0.0, -1.0j]), atol=1e-7)
+
+
+def test_expectation_kats(gpu):
+    """tests/test_circuit.py:239-245, 370-390, 415-429, 1278-1281, 1373-1380."""
+    import tensorcircuit_amd as tc
+
+    c = tc.Circuit(2)
+    c.H(0)
+    np.testing.assert_allclose(np_of(c.expectation((tc.gates.z(), [0]))), 0, atol=1e-7)
+    c = tc.Circuit(2)
+    c.X(0)
+    np.testing.assert_allclose(np_of(c.expectation_ps(z=[0, 1])), -1, atol=1e-5)
+    c = tc.Circuit(2)
+    c.H(0)
+    np.testing.assert_allclose(np_of(c.expectation_ps(z=[1], x=[0])), 1, atol=1e-5)
+    np.testing.assert_allclose(np_of(c.expectation_ps(ps=[1, 3])), 1, atol=1e-5)
+    np.testing.assert_allclose(np_of(c.expectation_ps(z=[1, 2], ps=[1, 3])), 1, atol=1e-5)
+    c = tc.Circuit(1, inputs=1 / np.sqrt(2) * np.array([-1, 1.0j]))
+    np.testing.assert_allclose(np_of(c.expectation_ps(y=[0])), -1, atol=1e-5)
+    c = tc.Circuit(3)
+    c.H(-2)
+    c.H(0)
+    np.testing.assert_allclose(np_of(c.expectation_ps(x=[0])).real, 1, atol=1e-5)
+    np.testing.assert_allclose(np_of(c.expectation_ps(x=[1])).real, 1, atol=1e-5)
+    np.testing.assert_allclose(np_of(c.expectation_ps(x=[-1])).real, 0, atol=1e-5)
+    np.testing.assert_allclose(np_of(c.expectation_ps(z=[-2])).real, 0, atol=1e-5)
+    for inp, want in [([0, 0, 0, 1.0], 1.0), ([0, 0, 1.0, 0], 1.0), ([1.0, 0, 0, 0], -1.0)]:
+        c = tc.Circuit(2, inputs=tc.array_to_tensor(np.array(inp)))
+        c.X(0)
+        assert np_of(c.expectation((tc.gates.z(), [0]))) == want
+
+
+def test_gate_kats(gpu):
+    """tests/test_gates.py:18-22, 80-90, 93-96, 111-124, 147-156."""
+    import tensorcircuit_amd as tc
+
+    c = tc.Circuit(1)
+    c.h(0)
+    c.phase(0, theta=np.pi / 2)
+    np.testing.assert_allclose(np_of(c.state())[1], 0.7071j, atol=1e-4)
+    c = tc.Circuit(2)
+    c.exp(0, 1, unitary=np.array([[1.0, 0, 0, 0], [0, -1.0, 0, 0], [0, 0, -1, 0], [0, 0, 0, 1]]),
+          theta=np.pi / 2)
+    np.testing.assert_allclose(np_of(c.wavefunction())[0], -1j, atol=1e-6)
+    c = tc.Circuit(2)
+    c.any(0, unitary=np.eye(2))
+    np.testing.assert_allclose(np_of(c.expectation((tc.gates.z(), [0]))), 1.0)
+    ocx = sv._octrl(sv.CONST_GATES["cnot"])
+    c = tc.Circuit(3)
+    c.x(0)
+    c.any(1, 0, 2, unitary=ocx)
+    np.testing.assert_allclose(np_of(c.expectation([tc.gates.z(), [2]])), -1, atol=1e-5)
+    c1 = tc.Circuit(3)
+    c1.rxx(0, 1, theta=1.0)
+    c1.ryy(0, 2, theta=0.5)
+    c1.rzz(0, 1, theta=-0.5)
+    c2 = tc.Circuit(3)
+    c2.exp1(0, 1, theta=1.0 / 2, unitary=tc.gates._xx_matrix)
+    c2.exp1(0, 2, theta=0.5 / 2, unitary=tc.gates._yy_matrix)
+    c2.exp1(0, 1, theta=-0.5 / 2, unitary=tc.gates._zz_matrix)
+    np.testing.assert_allclose(np_of(c1.state()), np_of(c2.state()), atol=1e-5)
+
+
+def test_unitary_kron_inputs(gpu):
+    """tests/test_circuit.py:404-412."""
+    import tensorcircuit_amd as tc
+
+    c = tc.Circuit(2, inputs=np.eye(4))
+    c.X(0)
+    c.Y(1)
+    np.testing.assert_allclose(np_of(c.wavefunction()).reshape(4, 4), np.kron(sv._X, sv._Y), atol=1e-5)
+
+
+# ------------------------------------------------------- random circuits ---
+
+NAMES_1Q = ["h", "x", "y", "z", "s", "t", "sd", "td", "rx", "ry", "rz", "phase", "wroot"]
+NAMES_2Q = ["cnot", "cz", "swap", "cy", "rzz", "rxx", "ryy", "iswap", "crx", "unitary2"]
+
+
+def _random_circuit(rng, n, ngates, with_3q=True):
+    ops = []
+    for _ in range(ngates):
+        r = rng.random()
+        if with_3q and n >= 3 and r < 0.04:
+            qs = tuple(int(q) for q in rng.permutation(n)[:3])
+            ops.append(("toffoli", qs, {}))
+        elif r < 0.55:
+            ops.append((str(rng.choice(NAMES_1Q)), (int(rng.integers(n)),), {"theta": float(rng.uniform(0, 2 * np.pi))}))
+        else:
+            qs = tuple(int(q) for q in rng.permutation(n)[:2])
+            name = str(rng.choice(NAMES_2Q))
+            kw = {"theta": float(rng.uniform(0, 2 * np.pi))}
+            if name == "unitary2":
+                u = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+                kw = {"unitary": u}
+            ops.append((name, qs, kw))
+    return ops
+
+
+def _build(tc, n, ops):
+    c = tc.Circuit(n)
+    for name, qs, kw in ops:
+        if name in ("h", "x", "y", "z", "s", "t", "sd", "td", "wroot", "cnot", "cz", "swap", "cy", "toffoli"):
+            getattr(c, name)(*qs)
+        elif name == "unitary2":
+            c.unitary(*qs, unitary=kw["unitary"])
+        else:
+            getattr(c, name)(*qs, theta=kw["theta"])
+    return c
+
+
+def _oracle_ops(ops):
+    out = []
+    for name, qs, kw in ops:
+        if name == "unitary2":
+            out.append(sv.Op("unitary", qs, kw={"unitary": kw["unitary"]}))
+        elif name in ("h", "x", "y", "z", "s", "t", "sd", "td", "wroot", "cnot", "cz", "swap", "cy", "toffoli"):
+            out.append(sv.Op(name, qs))
+        else:
+            out.append(sv.Op(name, qs, kw={"theta": kw["theta"]}))
+    return out
+
+
+@pytest.mark.parametrize("n", [3, 6, 10, 12, 14])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("fuse", [0, 1])
+def test_random_circuit_state(gpu, n, dtype, fuse):
+    import tensorcircuit_amd as tc
+    from tensorcircuit_amd import engine
+
+    tc.set_dtype(dtype)
+    rng = np.random.default_rng(1000 + n)
+    ops = _random_circuit(rng, n, 6 * n)
+    c = _build(tc, n, ops)
+    specs, thetas = c._lower()
+    prog = engine.get_program(n, specs, len(thetas), fuse=fuse)
+    params = c._params(thetas, gpu)
+    st, _ = prog.forward(params.reshape(1, -1), 1, dtype)
+    ref = sv.run_ops(n, _oracle_ops(ops))
+    tol = C64_TOL if dtype == "complex64" else C128_TOL
+    np.testing.assert_allclose(np_of(st).reshape(-1), ref, atol=tol * 4)
+    # the drop-in path (fused default) agrees too
+    np.testing.assert_allclose(np_of(c.wavefunction()), ref, atol=tol * 4)
+
+
+# --------------------------------------------------------------- HEA/TFIM ---
+
+def _hea_params(seed, L, n, batch=None):
+    rng = np.random.default_rng(seed)
+    shape = (2 * L, n) if batch is None else (batch, 2 * L, n)
+    return rng.uniform(0, 2 * np.pi, size=shape).astype(np.float32)
+
+
+def _tfim_codes(n, h=1.0):
+    terms = sv.tfim_terms(n, h)
+    return np.array([c for _, c in terms], dtype=np.int8), np.array([w for w, _ in terms])
+
+
+@pytest.mark.parametrize("n,L", [(12, 4), (16, 3)])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_hea_energy_and_grad(gpu, n, L, dtype):
+    """C1-shaped workload: fused adjoint energy + gradient vs the c128 oracle."""
+    import tensorcircuit_amd as tc
+    from tensorcircuit_amd import engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    tc.set_dtype(dtype)
+    p = _hea_params(1001, L, n)
+    c = tc.Circuit(n)
+    hea_layers(c, p, L)
+    specs, thetas = c._lower()
+    codes, coeffs = _tfim_codes(n)
+    ham = engine.get_pauli(n, codes, coeffs)
+    ops = sv.hea_ops(n, L)
+    e_ref, g_ref = sv.adjoint_value_and_grad(n, ops, p.astype(np.float64).reshape(-1), sv.tfim_terms(n))
+    tol = 1e-5 if dtype == "complex64" else C128_TOL
+    for fuse, ck in ((0, 0), (1, 0), (1, 2)):
+        prog = engine.get_program(n, specs, len(thetas), fuse=fuse, ckpt_every=ck)
+        e, g = prog.energy_grad(torch.as_tensor(p.reshape(1, -1)), ham, 1, dtype)
+        assert abs(np_of(e)[0] - e_ref) <= tol * max(1.0, abs(e_ref)), (fuse, ck, np_of(e)[0], e_ref)
+        assert np.max(np.abs(np_of(g).reshape(-1) - g_ref)) <= tol, (fuse, ck)
+
+
+def test_hea_batch_matches_single(gpu):
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L, B = 12, 3, 5
+    ps = _hea_params(7, L, n, batch=B)
+    c = Circuit(n)
+    hea_layers(c, ps[0], L)
+    specs, thetas = c._lower()
+    codes, coeffs = _tfim_codes(n)
+    ham = engine.get_pauli(n, codes, coeffs)
+    prog = engine.get_program(n, specs, len(thetas), fuse=1, ckpt_every=1)
+    e, g = prog.energy_grad(torch.as_tensor(ps.reshape(B, -1)), ham, B)
+    for b in range(B):
+        e1, g1 = prog.energy_grad(torch.as_tensor(ps[b].reshape(1, -1)), ham, 1)
+        np.testing.assert_allclose(np_of(e)[b], np_of(e1)[0], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(np_of(g)[b], np_of(g1)[0], rtol=0, atol=1e-12)
+        e_ref, g_ref = sv.adjoint_value_and_grad(n, sv.hea_ops(n, L), ps[b].astype(np.float64).reshape(-1),
+                                                 sv.tfim_terms(n))
+        assert abs(np_of(e)[b] - e_ref) < 1e-5
+        assert np.max(np.abs(np_of(g)[b] - g_ref)) < 1e-5
+
+
+def test_zero_angles_closed_form_24q(gpu):
+    """All-zero angles keep |0..0>: E = -(n-1) exactly; gradient of the
+    rz angles vanishes (|0..0> is an rz eigenstate)."""
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L = 24, 2
+    p = np.zeros((2 * L, n), dtype=np.float32)
+    c = Circuit(n)
+    hea_layers(c, p, L)
+    specs, thetas = c._lower()
+    codes, coeffs = _tfim_codes(n)
+    ham = engine.get_pauli(n, codes, coeffs)
+    prog = engine.get_program(n, specs, len(thetas), fuse=1, ckpt_every=1)
+    e, g = prog.energy_grad(torch.as_tensor(p.reshape(1, -1)), ham, 1)
+    assert abs(np_of(e)[0] + (n - 1)) < 1e-6
+    g = np_of(g).reshape(2 * L, n)
+    assert np.max(np.abs(g[1::2])) < 1e-6
+    st, _ = prog.forward(torch.as_tensor(p.reshape(1, -1)), 1, "complex64")
+    s = np_of(st).reshape(-1)
+    assert abs(s[0] - 1) < 1e-6 and np.max(np.abs(s[1:])) < 1e-7
+
+
+def test_norm_preserved_large(gpu):
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L = 22, 3
+    p = _hea_params(1002, L, n)
+    c = Circuit(n)
+    hea_layers(c, p, L)
+    specs, thetas = c._lower()
+    prog = engine.get_program(n, specs, len(thetas), fuse=1)
+    st, _ = prog.forward(torch.as_tensor(p.reshape(1, -1)), 1, "complex64")
+    nrm = engine.inner(st, st, n).real
+    assert abs(float(nrm[0]) - 1.0) < 1e-5
+
+
+# ------------------------------------------------------- drop-in autograd ---
+
+def test_interface_gradient_kats(gpu):
+    """tests/test_interfaces.py:30-97 through backend.grad on the Circuit API."""
+    import tensorcircuit_amd as tc
+
+    K = tc.backend
+    n = 4
+
+    def f(param):
+        c = tc.Circuit(n)
+        c = tc.templates.blocks.example_block(c, param)
+        return K.real(c.expectation([tc.gates.x(), [1]])) ** 2
+
+    g = K.grad(f)(torch.ones([4, n]))
+    np.testing.assert_allclose(np_of(g)[0, 1], -2.146e-3, atol=1e-5)
+
+    def f2(paramzz, paramx):
+        c = tc.Circuit(n)
+        for i in range(n):
+            c.H(i)
+        for j in range(2):
+            for i in range(n - 1):
+                c.exp1(i, i + 1, unitary=tc.gates._zz_matrix, theta=paramzz[j, i])
+            for i in range(n):
+                c.rx(i, theta=paramx[j, i])
+        return K.real(c.expectation([tc.gates.x(), [1]])) - K.real(c.expectation([tc.gates.x(), [2]]))
+
+    g1, g2 = K.grad(f2, argnums=(0, 1))(torch.ones([2, n]), torch.ones([2, n]))
+    np.testing.assert_allclose(np_of(g1)[0, 0], -0.41609, atol=1e-5)
+
+
+def _vqe_energy(inputs, param, n, nlayers):
+    """tests/test_backends.py:426-449."""
+    import tensorcircuit_amd as tc
+
+    c = tc.Circuit(n, inputs=inputs)
+    paramc = tc.backend.cast(param, "complex64")
+    for i in range(n):
+        c.H(i)
+    for j in range(nlayers):
+        for i in range(n - 1):
+            c.ryy(i, i + 1, theta=paramc[2 * j, i])
+        for i in range(n):
+            c.rx(i, theta=paramc[2 * j + 1, i])
+    e = 0.0
+    for i in range(n):
+        e += c.expectation((tc.gates.x(), [i]))
+    for i in range(n - 1):
+        e += c.expectation((tc.gates.z(), [i]), (tc.gates.z(), [(i + 1) % n]))
+    return tc.backend.real(e)
+
+
+def test_vvag_consistency(gpu):
+    """tests/test_backends.py:452-475: vvag == value_and_grad per sample."""
+    from functools import partial
+
+    import tensorcircuit_amd as tc
+
+    K = tc.backend
+    n, nlayers = 4, 3
+    inp = K.ones([2**n]) / 2 ** (n / 2)
+    param = K.ones([2 * nlayers, n])
+    f = partial(_vqe_energy, n=n, nlayers=nlayers)
+    v0, (g00, g01) = K.value_and_grad(f, argnums=(0, 1))(inp, param)
+    batch = 8
+    inps = K.cast(K.ones([batch, 2**n]) / 2 ** (n / 2), "complex64")
+    v1, (g10, g11) = K.vvag(f, argnums=(0, 1))(inps, param)
+    np.testing.assert_allclose(np_of(v1)[0], np_of(v0), atol=1e-4)
+    np.testing.assert_allclose(np_of(g10)[0], np_of(g00), atol=1e-4)
+    np.testing.assert_allclose(np_of(g11) / batch, np_of(g01), atol=1e-4)
+    # against the oracle: same energy function in complex128
+    ops = [sv.Op("h", (i,)) for i in range(n)]
+    for j in range(nlayers):
+        for i in range(n - 1):
+            ops.append(sv.Op("ryy", (i, i + 1), pidx=(2 * j) * n + i))
+        for i in range(n):
+            ops.append(sv.Op("rx", (i,), pidx=(2 * j + 1) * n + i))
+    terms = [(1.0, [1 if q == i else 0 for q in range(n)]) for i in range(n)]
+    terms += [(1.0, [3 if q in (i, i + 1) else 0 for q in range(n)]) for i in range(n - 1)]
+    e_ref, g_ref = sv.adjoint_value_and_grad(n, ops, np.ones(2 * nlayers * n), terms,
+                                             inputs=np.ones(2**n) / 2 ** (n / 2))
+    np.testing.assert_allclose(np_of(v0), e_ref, atol=1e-5)
+    np.testing.assert_allclose(np_of(g01).reshape(-1), g_ref, atol=1e-5)
+
+
+def test_grad_vs_finite_difference_c128(gpu, highp):
+    """Drop-in value_and_grad in complex128 vs central differences of the oracle."""
+    import tensorcircuit_amd as tc
+
+    K = tc.backend
+    n, L = 6, 2
+    p = torch.as_tensor(_hea_params(3, L, n).astype(np.float64))
+
+    def energy(param):
+        c = tc.Circuit(n)
+        tc.templates.blocks.hea_layers(c, param, L)
+        e = 0.0
+        for w, codes in sv.tfim_terms(n):
+            e = e + w * c.expectation_ps(ps=codes)
+        return K.real(e)
+
+    v, g = K.value_and_grad(energy)(p)
+    ops = sv.hea_ops(n, L)
+    fd = sv.finite_difference_grad(n, ops, p.numpy().reshape(-1), sv.tfim_terms(n), eps=1e-3)
+    ps = sv.parameter_shift_grad(n, ops, p.numpy().reshape(-1), sv.tfim_terms(n))
+    np.testing.assert_allclose(np_of(g).reshape(-1), ps, atol=1e-10)
+    np.testing.assert_allclose(np_of(g).reshape(-1), fd, atol=1e-5)
+    np.testing.assert_allclose(np_of(v), sv.energy(n, ops, p.numpy().reshape(-1), sv.tfim_terms(n)), atol=1e-12)
+
+
+# ----------------------------------------------------------------- Pauli ---
+
+def test_pauli_sums_vs_oracle(gpu):
+    from tensorcircuit_amd import engine
+
+    rng = np.random.default_rng(99)
+    for n, T in ((5, 9), (11, 20), (13, 7)):
+        s = rng.standard_normal(2**n) + 1j * rng.standard_normal(2**n)
+        s /= np.linalg.norm(s)
+        codes = rng.integers(0, 4, size=(T, n)).astype(np.int8)
+        w = rng.standard_normal(T)
+        ham = engine.PauliSum(n, codes, w)
+        terms = [(float(w[k]), list(codes[k])) for k in range(T)]
+        for dt in (torch.complex64, torch.complex128):
+            st = torch.as_tensor(s).to(dt).cuda()
+            e = np_of(ham.expect(st))[0]
+            ref = sv.pauli_sum_expectation(s.astype(np.complex64 if dt == torch.complex64 else np.complex128), n,
+                                           terms)
+            tol = 1e-5 if dt == torch.complex64 else 1e-12
+            assert abs(e - ref) < tol * np.abs(w).sum(), (n, dt, e, ref)
+            hs = np_of(ham.apply(st))[0]
+            np.testing.assert_allclose(hs, sv.pauli_sum_apply(s, n, terms), atol=tol * 10 * np.abs(w).sum())
+
+
+def test_heisenberg_ring_energy(gpu):
+    """tests/test_quantum.py:316-321: the 6-site ring ground energy -11.2111
+    from <psi|H|psi> on the exact ground state."""
+    from tensorcircuit_amd import engine
+
+    n = 6
+    codes, w = [], []
+    for a in range(n):
+        for code in (3, 1, 2):
+            c = [0] * n
+            c[a] = code
+            c[(a + 1) % n] = code
+            codes.append(c)
+            w.append(1.0)
+    hm = sum(wi * sv.pauli_string_matrix(ci) for wi, ci in zip(w, codes))
+    evals, evecs = np.linalg.eigh(hm)
+    ham = engine.PauliSum(n, codes, w)
+    e = np_of(ham.expect(torch.as_tensor(evecs[:, 0]).to(torch.complex128).cuda()))[0]
+    np.testing.assert_allclose(e.real, -11.2111, atol=1e-4)

@@ -1,0 +1,192 @@
+"""Host-side logic on the CPU (no GPU): the C ABI loads and exports every
+declared symbol, the scheduler's pass/group order is a valid reordering of
+the gate list (replayed through the oracle), argument errors map to the
+reference's exception types, and the Circuit facade records the reference's
+QIR."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import statevector as sv
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    from tensorcircuit_amd import _lib
+
+    hdr = open(os.path.join(ROOT, "include", "tcm.h")).read()
+    declared = set(re.findall(r"^(?:int|int64_t|const char\*)\s+(tcm_\w+)\(", hdr, re.M))
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(_lib.lib, name), name
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    assert _lib.lib.tcm_abi_version() == 1
+
+
+def _random_specs(rng, n, ngates, with_3q=False):
+    from tensorcircuit_amd import _lib
+    from tensorcircuit_amd.engine import GateSpec
+
+    specs, mats = [], []
+    for _ in range(ngates):
+        r = rng.random()
+        if with_3q and r < 0.05 and n >= 3:
+            qs = tuple(int(q) for q in rng.permutation(n)[:3])
+            m = sv.CONST_GATES["toffoli"]
+        elif r < 0.5:
+            qs = (int(rng.integers(n)),)
+            m = np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))[0]
+        else:
+            qs = tuple(int(q) for q in rng.permutation(n)[:2])
+            m = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+        specs.append(GateSpec(_lib.G_MAT, qs, -1, 0.0, 1.0, m))
+        mats.append((m, qs))
+    return specs, mats
+
+
+def _replay(desc, mats, n):
+    """Apply the gates in the order the schedule executes them."""
+    order = []
+    for p in desc["sched"]["passes"]:
+        if not p["groups"]:
+            order.extend(p["gates"])
+            continue
+        tile = p["tile"]
+        tile_set = set(tile)
+        gates_in_groups = []
+        for g in p["groups"]:
+            slots = set(tile[s] for s in g["slots"])
+            for gi in g["gates"]:
+                m, qs = mats[gi]
+                bits = {n - 1 - q for q in qs}
+                assert bits <= tile_set, "gate outside its tile"
+                assert bits <= slots, "gate outside its register group"
+            gates_in_groups.extend(g["gates"])
+        assert sorted(gates_in_groups) == sorted(p["gates"])
+        order.extend(gates_in_groups)
+    assert sorted(order) == list(range(len(mats)))
+    s = sv.zero_state(n)
+    rng = np.random.default_rng(0)
+    s = rng.standard_normal(2**n) + 1j * rng.standard_normal(2**n)
+    s0 = s.copy()
+    for gi in order:
+        m, qs = mats[gi]
+        s = sv.apply_gate(s, m, qs, n)
+    ref = s0
+    for m, qs in mats:
+        ref = sv.apply_gate(ref, m, qs, n)
+    return s, ref
+
+
+@pytest.mark.parametrize("n,k,R,dtype,mode", [
+    (10, 13, 4, "complex64", 0), (12, 10, 4, "complex64", 0), (14, 10, 4, "complex64", 1),
+    (16, 12, 4, "complex128", 0), (13, 11, 4, "complex128", 1), (20, 13, 4, "complex64", 1)])
+def test_schedule_is_a_valid_reordering(n, k, R, dtype, mode):
+    from tensorcircuit_amd.engine import Program
+
+    rng = np.random.default_rng(n * 100 + k)
+    specs, mats = _random_specs(rng, n, 80, with_3q=True)
+    prog = Program(n, specs, 0, fuse=1, tile_bits=k, adj_tile_bits=k, reg_slots=R)
+    desc = prog.describe(dtype, mode)
+    assert desc["sched"]["k"] == min(k, n)
+    # every pass has its device descriptor (a missing one crashed the launcher)
+    assert desc["sched"]["ndpass"] == len(desc["sched"]["passes"])
+    got, ref = _replay(desc, mats, n)
+    np.testing.assert_allclose(got, ref, atol=1e-10)
+
+
+def test_hea_schedule_shape():
+    """24-qubit HEA layer packs into about two tile passes per layer."""
+    from tensorcircuit_amd import Circuit
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L = 24, 4
+    c = Circuit(n)
+    hea_layers(c, np.zeros((2 * L, n)), L)
+    specs, thetas = c._lower()
+    from tensorcircuit_amd.engine import Program
+
+    prog = Program(n, specs, len(thetas), fuse=1, ckpt_every=2)
+    d = prog.describe("complex64", 0)
+    npass = len(d["sched"]["passes"])
+    assert npass <= 3 * L, npass
+    assert prog.nckpt == 1  # after layer 2 (the barrier after the last layer is not a checkpoint)
+    da = prog.describe("complex64", 1)
+    assert len(da["sched"]["ckpt_after"]) == 1
+    assert da["sched"]["ndg"] > 0  # rz runs merged into group diagonals
+
+
+def test_plain_schedule_one_pass_per_gate():
+    from tensorcircuit_amd import Circuit
+    from tensorcircuit_amd.engine import Program
+
+    c = Circuit(5)
+    c.h(0)
+    c.rx(1, theta=0.1)
+    c.cnot(0, 4)
+    specs, thetas = c._lower()
+    prog = Program(5, specs, len(thetas), fuse=0)
+    d = prog.describe("complex64", 0)
+    assert d["sched"]["k"] == 0 and len(d["sched"]["passes"]) == 3
+
+
+def test_argument_errors_map_to_exceptions():
+    from tensorcircuit_amd import _lib
+    from tensorcircuit_amd.engine import GateSpec, PauliSum, Program
+
+    with pytest.raises(ValueError):
+        Program(3, [GateSpec(_lib.G_MAT, (0, 0), -1, 0.0, 1.0, np.eye(4))], 0)
+    with pytest.raises(ValueError):
+        Program(3, [GateSpec(_lib.G_MAT, (3,), -1, 0.0, 1.0, np.eye(2))], 0)
+    with pytest.raises(ValueError):
+        Program(3, [GateSpec(_lib.G_RX, (0,), 2, 0.0, 1.0, None)], 1)  # param out of range
+    with pytest.raises(ValueError):
+        Program(3, [GateSpec(_lib.G_MAT, (0,), -1, 0.0, 1.0, np.array([[np.nan, 0], [0, 1]]))], 0)
+    with pytest.raises(NotImplementedError):  # trainable exp1 needs G^2 = I
+        Program(2, [GateSpec(_lib.G_EXP1, (0, 1), 0, 0.0, 1.0, np.diag([1.0, 2, 3, 4]))], 1)
+    with pytest.raises(NotImplementedError):
+        Program(12, [GateSpec(_lib.G_MAT, (0,), -1, 0.0, 1.0, np.eye(2))], 0, reg_slots=5)
+    with pytest.raises(ValueError):
+        PauliSum(2, [[0, 5]], [1.0])
+    with pytest.raises(ValueError):
+        PauliSum(2, [[0, 1]], [np.inf])
+
+
+def test_compute_without_device_raises():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from tensorcircuit_amd import Circuit
+
+    c = Circuit(2)
+    c.h(0)
+    with pytest.raises(RuntimeError):
+        c.wavefunction()
+
+
+def test_circuit_records_reference_qir():
+    from tensorcircuit_amd import Circuit
+
+    c = Circuit(4)
+    c.cx([0, 1], [-1, -2])
+    c.h(range(4))
+    c.rz([0], theta=0.2)
+    c.rx([1, 2], theta=[0.3, 0.5])
+    c.rzz([2, 3], [0, 1], theta=np.ones([2]))
+    c.rxx([2, 0, 1], [0, 1, 2], theta=np.ones([1]))
+    assert c.gate_count("h") == 4
+    assert c.gate_count("rzz") == 2
+    assert c.gate_count("rxx") == 3
+    qir = c.to_qir()
+    assert qir[0]["index"] == (0, 3) and qir[0]["name"] == "cnot"
+    assert qir[8]["parameters"]["theta"] == 0.5
+    with pytest.raises(AssertionError):
+        c.cnot(1, 1)
+    with pytest.raises(ValueError):
+        c.expectation_ps(x=[0], z=[0])
