@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--kernel-only", choices=["fwd", "adj", "copy"], default=None,
+                    help="run only the fused forward passes, the adjoint sweep, or a state copy of the same "
+                         "size (for counter profiles); prints no JSON")
     return ap.parse_args()
 
 
@@ -77,6 +80,18 @@ def main():
                           adj_tile_bits=args.adj_tile)
     rng = np.random.default_rng(1004)
     params = torch.as_tensor(rng.uniform(0, 2 * np.pi, size=(B, P)).astype(np.float32)).to(dev)
+    if args.kernel_only:
+        st, ck = prog.forward(params, B, "complex64", want_ckpts=True)
+        for _ in range(max(args.steps, 1)):
+            if args.kernel_only == "fwd":
+                prog.forward(params, B, "complex64", out=st)
+            elif args.kernel_only == "adj":
+                prog.vjp(params, ck, st, st, B, "complex64", False)
+            else:
+                torch.empty_like(st).copy_(st)
+        torch.cuda.synchronize()
+        print("kernel-only %s done" % args.kernel_only, file=sys.stderr)
+        return
     e, g = prog.energy_grad(params, ham, B)
     torch.cuda.synchronize()
     staging_s = time.perf_counter() - t_stage

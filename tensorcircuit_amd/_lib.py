@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtcm.so")
+# TCM_LIBRARY selects another build of the same library (e.g. a debug build)
+LIB_PATH = os.environ.get("TCM_LIBRARY") or os.path.join(HERE, "libtcm.so")
 
 TCM_C64 = 0
 TCM_C128 = 1

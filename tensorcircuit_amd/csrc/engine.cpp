@@ -20,13 +20,30 @@ struct tcm_program {
 
 struct PauliDev {
   void* terms = nullptr;
+  void* tpasses[2] = {nullptr, nullptr};
+  void* tgroups[2] = {nullptr, nullptr};
+  void* tterms[2] = {nullptr, nullptr};
+  void* truns[2] = {nullptr, nullptr};
+};
+
+// LDS-tiled schedule of a Pauli sum for one dtype
+struct PauliTileSched {
+  bool ok = false;                 // false: every flip group runs the per-group kernel
+  std::vector<DPPass> passes;
+  std::vector<DPGroupT> groups;
+  std::vector<DPTermT> terms;
+  std::vector<uint32_t> runoff;
+  std::vector<int> legacy;         // flip groups wider than a tile
+  std::vector<int> legacy_slot0;
+  int nslots = 0;                  // energy-partial slots of this dtype
 };
 
 struct tcm_pauli {
   int n = 0;
-  std::vector<DPGroup> groups;
+  std::vector<DPGroup> groups;     // by flip mask, diagonal first
   std::vector<DPTerm> terms;
-  int nslots = 0;
+  int nslots = 0;                  // slots of the per-group kernels
+  PauliTileSched ts[2];
   std::mutex mu;
   std::vector<PauliDev> dev;
 };
@@ -207,17 +224,28 @@ int forward_impl(Program& P, int dtype, const void* params, int64_t pstride, voi
 }
 
 // ---- Pauli -----------------------------------------------------------------
-int ensure_pauli(tcm_pauli* H, const DPTerm** terms_dev) {
+int ensure_pauli(tcm_pauli* H, int dt, PauliDev** out) {
   int dev = 0;
   if (int rc = current_device(&dev)) return rc;
   std::lock_guard<std::mutex> lk(H->mu);
   if ((int)H->dev.size() <= dev) H->dev.resize(dev + 1);
-  if (!H->dev[dev].terms) {
-    if (int rc = upload(H->terms, &H->dev[dev].terms)) return rc;
+  PauliDev& pd = H->dev[dev];
+  if (!pd.terms) {
+    if (int rc = upload(H->terms, &pd.terms)) return rc;
   }
-  *terms_dev = (const DPTerm*)H->dev[dev].terms;
+  const PauliTileSched& S = H->ts[dt];
+  if (S.ok && !pd.tpasses[dt]) {
+    int rc = upload(S.passes, &pd.tpasses[dt]);
+    rc = rc ? rc : upload(S.groups, &pd.tgroups[dt]);
+    rc = rc ? rc : upload(S.terms, &pd.tterms[dt]);
+    rc = rc ? rc : upload(S.runoff, &pd.truns[dt]);
+    if (rc) return rc;
+  }
+  *out = &pd;
   return 0;
 }
+
+int pauli_slots(const tcm_pauli* H, int dt) { return H->ts[dt].ok ? H->ts[dt].nslots : H->nslots; }
 
 int pauli_blocks(int n, bool flip) {
   const int64_t items = flip ? (1ll << (n - 1)) : (1ll << n);
@@ -225,19 +253,131 @@ int pauli_blocks(int n, bool flip) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(1024, b));
 }
 
-int64_t pauli_ws(const tcm_pauli* H, int64_t batch) {
-  return align_up(batch * (int64_t)H->nslots * 2 * sizeof(double)) + align_up(batch * 2 * sizeof(double));
+int64_t pauli_ws(const tcm_pauli* H, int dt, int64_t batch) {
+  return align_up(batch * (int64_t)pauli_slots(H, dt) * 2 * sizeof(double)) + align_up(batch * 2 * sizeof(double));
 }
 
 template <typename T>
-int pauli_energy_apply(tcm_pauli* H, const void* psi, void* out, const double* alpha, int64_t batch, double* epart,
-                       bool energy, bool apply, hipStream_t st) {
-  const DPTerm* td = nullptr;
-  if (int rc = ensure_pauli(H, &td)) return rc;
-  for (size_t g = 0; g < H->groups.size(); ++g)
-    launch_pauli<T>(H->groups[g], td, H->n, psi, out, g == 0 ? 1 : 0, alpha, epart, (int64_t)H->nslots * 2, batch,
-                    energy, apply, st);
+int pauli_energy_apply(tcm_pauli* H, int dt, const void* psi, void* out, const double* alpha, int64_t batch,
+                       double* epart, bool energy, bool apply, hipStream_t st) {
+  PauliDev* pd = nullptr;
+  if (int rc = ensure_pauli(H, dt, &pd)) return rc;
+  const DPTerm* td = (const DPTerm*)pd->terms;
+  const int64_t estride = (int64_t)pauli_slots(H, dt) * 2;
+  const PauliTileSched& S = H->ts[dt];
+  if (!S.ok) {
+    for (size_t g = 0; g < H->groups.size(); ++g)
+      launch_pauli<T>(H->groups[g], td, H->n, psi, out, g == 0 ? 1 : 0, alpha, epart, estride, batch, energy, apply,
+                      st);
+    return 0;
+  }
+  bool first = true;
+  for (size_t pi = 0; pi < S.passes.size(); ++pi) {
+    launch_pauli_tile<T>(S.passes[pi], (const DPPass*)pd->tpasses[dt], (int)pi, (const DPGroupT*)pd->tgroups[dt],
+                         (const DPTermT*)pd->tterms[dt], (const uint32_t*)pd->truns[dt], psi, first ? nullptr : out,
+                         out, H->n, alpha, epart, estride, batch, energy, apply, st);
+    first = false;
+  }
+  for (size_t i = 0; i < S.legacy.size(); ++i) {
+    DPGroup G = H->groups[S.legacy[i]];
+    G.slot0 = S.legacy_slot0[i];
+    launch_pauli<T>(G, td, H->n, psi, out, first ? 1 : 0, alpha, epart, estride, batch, energy, apply, st);
+    first = false;
+  }
   return 0;
+}
+
+// pext: gather the bits of x at positions pos[0..] into bits 0..
+uint32_t pext_bits(uint64_t x, const std::vector<int>& pos) {
+  uint32_t r = 0;
+  for (size_t i = 0; i < pos.size(); ++i)
+    if ((x >> pos[i]) & 1) r |= 1u << i;
+  return r;
+}
+
+// Pack flip groups into LDS tiles: a tile = the low bits (coalescing) plus
+// the X/Y positions of the groups it evaluates, padded with the lowest free
+// bits; a group wider than a tile keeps the per-group kernel.
+void pauli_tile_sched(tcm_pauli* H, int dt) {
+  PauliTileSched& S = H->ts[dt];
+  const int n = H->n;
+  if (n < 10) {
+    S.ok = false;
+    return;
+  }
+  S.ok = true;
+  const int k = std::min(n, dt == 0 ? 12 : 11);
+  const int Lw = std::min(dt == 0 ? 4 : 3, k);
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+  const uint64_t lmask = (1ull << Lw) - 1;
+  std::vector<int> rem;
+  for (int g = 0; g < (int)H->groups.size(); ++g) {
+    if (__builtin_popcountll(H->groups[g].flip | lmask) > k) S.legacy.push_back(g);
+    else rem.push_back(g);
+  }
+  int slot = 0;
+  while (!rem.empty()) {
+    uint64_t tm = (n <= k) ? all : lmask;
+    std::vector<int> take, rest;
+    for (int g : rem) {
+      uint64_t f = H->groups[g].flip;
+      if (__builtin_popcountll(tm | f) <= k) {
+        tm |= f;
+        take.push_back(g);
+      } else {
+        rest.push_back(g);
+      }
+    }
+    for (int b = 0; b < n && __builtin_popcountll(tm) < k; ++b) tm |= 1ull << b;
+    std::vector<int> tile;
+    for (int b = 0; b < n; ++b)
+      if ((tm >> b) & 1) tile.push_back(b);
+    DPPass pp{};
+    pp.k = (int)tile.size();
+    int L = 0;
+    while (L < pp.k && tile[L] == L) ++L;
+    pp.L = L;
+    pp.nblk_log = n - pp.k;
+    int nb = 0;
+    for (int b = 0; b < n; ++b)
+      if (!((tm >> b) & 1)) pp.blk_bits[nb++] = b;
+    pp.runoff0 = (int)S.runoff.size();
+    for (int r = 0; r < (1 << (pp.k - L)); ++r) {
+      uint32_t off = 0;
+      for (int i = 0; i < pp.k - L; ++i)
+        if ((r >> i) & 1) off |= 1u << tile[L + i];
+      S.runoff.push_back(off);
+    }
+    pp.g0 = (int)S.groups.size();
+    pp.ng = (int)take.size();
+    pp.nthr_log = std::min(8, pp.k - 4);
+    pp.slot0 = slot;
+    slot += 1 << pp.nblk_log;
+    for (int g : take) {
+      const DPGroup& G = H->groups[g];
+      DPGroupT gt{};
+      gt.flip_t = pext_bits(G.flip, tile);
+      gt.t0 = (int)S.terms.size();
+      gt.nt = G.nt;
+      for (int t = G.t0; t < G.t0 + G.nt; ++t) {
+        const DPTerm& tm0 = H->terms[t];
+        DPTermT tt{};
+        tt.yz_t = pext_bits(tm0.yz & tm, tile);
+        tt.yz_g = tm0.yz & ~tm;
+        tt.ny = tm0.ny;
+        tt.w = tm0.w;
+        S.terms.push_back(tt);
+      }
+      S.groups.push_back(gt);
+    }
+    S.passes.push_back(pp);
+    rem.swap(rest);
+  }
+  for (int g : S.legacy) {
+    S.legacy_slot0.push_back(slot);
+    slot += H->groups[g].nblk;
+  }
+  S.nslots = slot;
 }
 
 // ---- adjoint ---------------------------------------------------------------
@@ -263,7 +403,7 @@ AdjWs carve_adj(const Program& P, const tcm_pauli* H, int dtype, int64_t batch, 
   w.part = (double*)cv.take(batch * ngp * (int64_t)A.max_slots * sizeof(double));
   w.gsum = (double*)cv.take(batch * (int64_t)std::max(P.ngop, 1) * sizeof(double));
   if (mode == 0 && H) {
-    w.epart = (double*)cv.take(batch * (int64_t)H->nslots * 2 * sizeof(double));
+    w.epart = (double*)cv.take(batch * (int64_t)pauli_slots(H, dtype) * 2 * sizeof(double));
     w.e2 = (double*)cv.take(batch * 2 * sizeof(double));
   } else {
     w.epart = w.e2 = nullptr;
@@ -361,8 +501,8 @@ int energy_grad_impl(Program& P, int dtype, const void* params, int64_t pstride,
   else launch_init_zero<T>(w.psi, P.n, batch, st);
   if (int rc = run_forward_passes<T>(P, F, *DF, w.mtab_f, w.psi, batch, w.ckpt, st)) return rc;
   // energy and lambda = H psi
-  if (int rc = pauli_energy_apply<T>(H, w.psi, w.lam, nullptr, batch, w.epart, true, true, st)) return rc;
-  launch_sum_slots(w.epart, (int64_t)H->nslots * 2, H->nslots, 2, w.e2, batch, st);
+  if (int rc = pauli_energy_apply<T>(H, dtype, w.psi, w.lam, nullptr, batch, w.epart, true, true, st)) return rc;
+  launch_sum_slots(w.epart, (int64_t)pauli_slots(H, dtype) * 2, pauli_slots(H, dtype), 2, w.e2, batch, st);
   // reverse sweep
   launch_prep<T>(A, *DA, DA->ctab, params, pstride, w.mtab_a, batch, st);
   if (int rc = reverse_sweep<T>(P, dtype, w, w.psi, w.lam, w.ckpt, nullptr, batch, st)) return rc;
@@ -537,6 +677,8 @@ int tcm_pauli_build(const int8_t* codes, const double* coeffs, int32_t nterms, i
     H->groups.push_back(g);
   }
   H->nslots = slot;
+  pauli_tile_sched(H, 0);
+  pauli_tile_sched(H, 1);
   *out = H;
   return 0;
 }
@@ -545,11 +687,16 @@ int tcm_pauli_free(tcm_pauli* ham) {
   if (!ham) return 0;
   int cur = -1;
   bool have = hipGetDevice(&cur) == hipSuccess;
-  for (size_t d = 0; d < ham->dev.size(); ++d)
-    if (ham->dev[d].terms) {
-      (void)hipSetDevice((int)d);
-      (void)hipFree(ham->dev[d].terms);
+  for (size_t d = 0; d < ham->dev.size(); ++d) {
+    PauliDev& pd = ham->dev[d];
+    (void)hipSetDevice((int)d);
+    if (pd.terms) (void)hipFree(pd.terms);
+    for (int dt = 0; dt < 2; ++dt) {
+      void* ps[] = {pd.tpasses[dt], pd.tgroups[dt], pd.tterms[dt], pd.truns[dt]};
+      for (void* q : ps)
+        if (q) (void)hipFree(q);
     }
+  }
   if (have) (void)hipSetDevice(cur);
   (void)hipGetLastError();
   delete ham;
@@ -558,7 +705,7 @@ int tcm_pauli_free(tcm_pauli* ham) {
 
 int tcm_pauli_workspace_bytes(const tcm_pauli* ham, int32_t dtype, int64_t batch, int64_t* bytes) {
   if (!ham || !bytes || !valid_dtype(dtype) || batch < 0) return set_error(TCM_E_ARG, "bad arguments");
-  *bytes = pauli_ws(ham, batch);
+  *bytes = pauli_ws(ham, dtype, batch);
   return 0;
 }
 
@@ -567,14 +714,15 @@ int tcm_expect_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
   if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out)))
     return set_error(TCM_E_ARG, "bad arguments to tcm_expect_pauli_sum");
   if (batch == 0) return 0;
-  if (pauli_ws(ham, batch) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  if (pauli_ws(ham, dtype, batch) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
   hipStream_t st = (hipStream_t)stream;
   tcm_pauli* H = const_cast<tcm_pauli*>(ham);
   double* epart = (double*)workspace;
-  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, state, nullptr, nullptr, batch, epart, true, false, st)
-                            : pauli_energy_apply<double>(H, state, nullptr, nullptr, batch, epart, true, false, st);
+  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, dtype, state, nullptr, nullptr, batch, epart, true, false, st)
+                            : pauli_energy_apply<double>(H, dtype, state, nullptr, nullptr, batch, epart, true, false, st);
   if (rc) return rc;
-  launch_sum_slots(epart, (int64_t)H->nslots * 2, H->nslots, 2, out, batch, st);
+  const int ns = pauli_slots(H, dtype);
+  launch_sum_slots(epart, (int64_t)ns * 2, ns, 2, out, batch, st);
   HIP_TRY(hipGetLastError(), "expect launch");
   return 0;
 }
@@ -586,8 +734,8 @@ int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state, 
   if (batch == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   tcm_pauli* H = const_cast<tcm_pauli*>(ham);
-  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, state, out_state, alpha, batch, nullptr, false, true, st)
-                            : pauli_energy_apply<double>(H, state, out_state, alpha, batch, nullptr, false, true, st);
+  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, dtype, state, out_state, alpha, batch, nullptr, false, true, st)
+                            : pauli_energy_apply<double>(H, dtype, state, out_state, alpha, batch, nullptr, false, true, st);
   if (rc) return rc;
   HIP_TRY(hipGetLastError(), "apply launch");
   return 0;

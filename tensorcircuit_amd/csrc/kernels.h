@@ -35,6 +35,26 @@ struct DPTerm {
   double w;        // weight
 };
 
+// LDS-tiled Pauli passes: every term whose X/Y positions lie in the tile is
+// evaluated from one read of psi
+struct DPPass {
+  int32_t k, L, nblk_log, runoff0;
+  int32_t g0, ng;      // groups of this pass
+  int32_t nthr_log;    // threads per block (log2)
+  int32_t slot0;       // first energy-partial slot
+  int32_t blk_bits[32];
+};
+struct DPGroupT {
+  uint32_t flip_t;     // tile-local X/Y mask
+  int32_t t0, nt, pad;
+};
+struct DPTermT {
+  uint64_t yz_g;       // Y/Z positions outside the tile (global bits)
+  uint32_t yz_t;       // Y/Z positions inside the tile (tile-local bits)
+  int32_t ny;
+  double w;
+};
+
 // device copies of one schedule's descriptors (owned by the program)
 struct DeviceTables {
   void* dpass = nullptr;
@@ -81,6 +101,11 @@ template <typename T>
 void launch_pauli(const DPGroup& G, const DPTerm* terms_dev, int n, const void* psi, void* out, int first,
                   const double* alpha, double* epart, int64_t estride, int64_t batch, bool energy, bool apply,
                   hipStream_t st);
+template <typename T>
+void launch_pauli_tile(const DPPass& P, const DPPass* passes_dev, int pi, const DPGroupT* groups_dev,
+                       const DPTermT* terms_dev, const uint32_t* runoff_dev, const void* psi, const void* lam_in,
+                       void* lam_out, int n, const double* alpha, double* epart, int64_t estride, int64_t batch,
+                       bool energy, bool apply, hipStream_t st);
 template <typename T>
 void launch_inner(const void* a, const void* c, int n, int64_t batch, int nblk, double* part, int64_t pstride,
                   hipStream_t st);

@@ -55,12 +55,34 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// one DPP row-shift of a double (lanes without a source read 0)
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double read_lane(double x, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                          __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+// fp64 sum over a full 64-lane wave, uniform result: inclusive row prefix
+// sums by DPP row_shr 1/2/4/8 leave each 16-lane row's total in its last
+// lane; the four row totals are read back as scalars.
+__device__ __forceinline__ double wave_sum_dpp(double x) {
+  x += dpp_shr<0x111>(x);
+  x += dpp_shr<0x112>(x);
+  x += dpp_shr<0x114>(x);
+  x += dpp_shr<0x118>(x);
+  return (read_lane(x, 15) + read_lane(x, 31)) + (read_lane(x, 47) + read_lane(x, 63));
+}
+
 // block-wide sum of (a, b) in fp64; result valid in thread 0.  blockDim.x
 // must be a multiple of 64 and <= 1024.
 __device__ __forceinline__ void block_sum2(double& a, double& b) {
   __shared__ double red[2][16];
-  a = wave_sum(a);
-  b = wave_sum(b);
+  a = wave_sum_dpp(a);
+  b = wave_sum_dpp(b);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
     red[0][w] = a;
@@ -268,8 +290,10 @@ __device__ __forceinline__ void ap_dg(V2<T>* v, const T* d) {
   }
 }
 
-// apply one op (kinds with a register form) on v
-template <typename T, int R, bool DAG>
+// apply one op (kinds with a register form) on v.  FULL = false drops the
+// dense 4x4 kind, whose 32-value matrix dominates register pressure; passes
+// without it run a leaner kernel with higher occupancy.
+template <typename T, int R, bool DAG, bool FULL = true>
 __device__ __forceinline__ void apply_op(V2<T>* v, const DOp& op, const T* mt, const T* ct) {
   switch (op.kind) {
     case OK_U1: {
@@ -288,8 +312,10 @@ __device__ __forceinline__ void apply_op(V2<T>* v, const DOp& op, const T* mt, c
       TCM_SW1(R, op.s0, ap_x1<T, R, S_>(v))
       break;
     case OK_U2: {
-      const T* m = mptr<T>(op.moff, mt, ct);
-      TCM_SW2(R, op.s0, op.s1, ap_u2<T, R, A_, B_, DAG>(v, m))
+      if constexpr (FULL) {
+        const T* m = mptr<T>(op.moff, mt, ct);
+        TCM_SW2(R, op.s0, op.s1, ap_u2<T, R, A_, B_, DAG>(v, m))
+      }
     } break;
     case OK_D2: {
       const T* m = mptr<T>(op.moff, mt, ct);
@@ -425,7 +451,7 @@ __device__ __forceinline__ T g_m2(const V2<T>* p, const V2<T>* l, const T* g) {
   return a;
 }
 
-template <typename T, int R>
+template <typename T, int R, bool FULL = true>
 __device__ __forceinline__ T grad_op(const V2<T>* p, const V2<T>* l, int gen, int s0, int s1, const T* ct, int goff) {
   T a = 0;
   switch (gen) {
@@ -437,7 +463,9 @@ __device__ __forceinline__ T grad_op(const V2<T>* p, const V2<T>* l, int gen, in
     case GN_YY: TCM_SW2(R, s0, s1, a = (g_yy<T, R, A_, B_>(p, l))) break;
     case GN_ZZ: TCM_SW2(R, s0, s1, a = (g_zz<T, R, A_, B_>(p, l))) break;
     case GN_M1: TCM_SW1(R, s0, a = (g_m1<T, R, S_>(p, l, ct + goff))) break;
-    case GN_M2: TCM_SW2(R, s0, s1, a = (g_m2<T, R, A_, B_>(p, l, ct + goff))) break;
+    case GN_M2:
+      if constexpr (FULL) { TCM_SW2(R, s0, s1, a = (g_m2<T, R, A_, B_>(p, l, ct + goff))) }
+      break;
     default: break;
   }
   return a;
@@ -758,7 +786,7 @@ __device__ __forceinline__ uint64_t block_base(const DPass* P, uint64_t blk) {
   return base;
 }
 
-template <typename T, int R, int TB>
+template <typename T, int R, int TB, bool FULL>
 __global__ void __launch_bounds__(TB) k_fused_fwd(const DPass* __restrict__ passes, int pi,
                                                     const DGroup* __restrict__ groups, const DOp* __restrict__ ops,
                                                     const uint32_t* __restrict__ runoff, const T* __restrict__ mtab,
@@ -792,7 +820,7 @@ __global__ void __launch_bounds__(TB) k_fused_fwd(const DPass* __restrict__ pass
     V2<T> v[1 << R];
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) v[j] = tile[a0 ^ (uint32_t)G->S[j]];
-    for (int o = 0; o < G->nops; ++o) apply_op<T, R, false>(v, ops[G->op0 + o], mt, ct);
+    for (int o = 0; o < G->nops; ++o) apply_op<T, R, false, FULL>(v, ops[G->op0 + o], mt, ct);
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) tile[a0 ^ (uint32_t)G->S[j]] = v[j];
     __syncthreads();
@@ -813,7 +841,7 @@ __global__ void __launch_bounds__(TB) k_fused_fwd(const DPass* __restrict__ pass
 // pass's trainable gates are reduced per tile into part[b][gloc][blk], and
 // the tiles are written to psi_out (skipped when NULL: a checkpoint replaces
 // it) and lam_out.
-template <typename T, int R, int TB>
+template <typename T, int R, int TB, bool FULL>
 __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ passes, int pi,
                                                     const DGroup* __restrict__ groups, const DOp* __restrict__ ops,
                                                     const DMember* __restrict__ mems,
@@ -827,6 +855,7 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
   const int k = P->k, L = P->L, nbl = P->nblk_log;
   V2<T>* tp = reinterpret_cast<V2<T>*>(smem);
   V2<T>* tl = tp + (1 << k);
+  // red[gloc][wave]: fp64 wave sums of each trainable gate of the pass
   double* red = reinterpret_cast<double*>(tl + (1 << k));
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = tid & 63, wv = tid >> 6, nw = nthr >> 6;
@@ -868,15 +897,15 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
         for (int j = 0; j < (1 << R); ++j) im[j] = imcj<T>(l[j], p[j]);
         for (int q = 0; q < op.nmem; ++q) {
           const DMember& M = mems[op.mem0 + q];
-          double gsum = wave_sum((double)grad_member<T, R>(im, M.gen, M.s0, M.s1));
-          if (lane == 0) red[M.gloc * nw + wv] = gsum;
+          const double gs = wave_sum_dpp((double)grad_member<T, R>(im, M.gen, M.s0, M.s1));
+          if (lane == 0) red[M.gloc * nw + wv] = gs;
         }
       } else if (op.gen != GN_NONE) {
-        double gsum = wave_sum((double)grad_op<T, R>(p, l, op.gen, op.s0, op.s1, ct, op.goff));
-        if (lane == 0) red[op.gloc * nw + wv] = gsum;
+        const double gs = wave_sum_dpp((double)grad_op<T, R, FULL>(p, l, op.gen, op.s0, op.s1, ct, op.goff));
+        if (lane == 0) red[op.gloc * nw + wv] = gs;
       }
-      apply_op<T, R, true>(p, op, mt, ct);
-      apply_op<T, R, true>(l, op, mt, ct);
+      apply_op<T, R, true, FULL>(p, op, mt, ct);
+      apply_op<T, R, true, FULL>(l, op, mt, ct);
     }
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
@@ -1039,6 +1068,83 @@ __global__ void __launch_bounds__(256) k_pauli(DPGroup G, const DPTerm* __restri
   }
 }
 
+// LDS-tiled Pauli pass.  For every amplitude a of the tile:
+//   h[a] = sum_groups sum_terms w i^ny (-1)^{popcount((y ^ flip) & yz)} psi[y ^ flip]
+// (the index rule of ps2coo_core, quantum.py:1461-1485), with the partner
+// read from LDS; energy += conj(psi[a]) h[a] in fp64 and, when APPLY,
+// lam_out[a] = lam_in[a] (or 0) + alpha * h[a].
+template <typename T, bool ENERGY, bool APPLY>
+__global__ void __launch_bounds__(256) k_pauli_tile(const DPPass* __restrict__ passes, int pi,
+                                                    const DPGroupT* __restrict__ groups,
+                                                    const DPTermT* __restrict__ terms,
+                                                    const uint32_t* __restrict__ runoff,
+                                                    const V2<T>* __restrict__ psi, const V2<T>* lam_in,
+                                                    V2<T>* lam_out, int64_t sstride,
+                                                    const double* __restrict__ alpha,
+                                                    double* __restrict__ epart, int64_t estride) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  V2<T>* tile = reinterpret_cast<V2<T>*>(smem);
+  const DPPass* P = passes + pi;
+  const int k = P->k, L = P->L, nbl = P->nblk_log;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
+  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
+  uint64_t base = 0;
+  for (int i = 0; i < nbl; ++i)
+    if ((blk >> i) & 1) base |= 1ull << P->blk_bits[i];
+  const V2<T>* s = psi + b * sstride + base;
+  const uint32_t* ro = runoff + P->runoff0;
+  const uint32_t lmask = (1u << L) - 1;
+  const int per = (1 << k) / nthr;
+  for (int it = 0; it < per; ++it) {
+    const uint32_t a = it * nthr + tid;
+    tile[swz(a)] = s[ro[a >> L] + (a & lmask)];
+  }
+  __syncthreads();
+  const double al = (APPLY && alpha) ? alpha[b] : 1.0;
+  double er = 0.0, ei = 0.0;
+  for (int it = 0; it < per; ++it) {
+    const uint32_t a = it * nthr + tid;
+    const V2<T> p = tile[swz(a)];
+    T hr = 0, hi = 0;
+    for (int g = P->g0; g < P->g0 + P->ng; ++g) {
+      const DPGroupT G = groups[g];
+      const uint32_t src = a ^ G.flip_t;
+      const V2<T> q = G.flip_t ? tile[swz(src)] : p;
+      T cr = 0, ci = 0;
+      for (int t = G.t0; t < G.t0 + G.nt; ++t) {
+        const DPTermT tm = terms[t];
+        const int par = (__popc(src & tm.yz_t) + __popcll(base & tm.yz_g)) & 1;
+        const T w = par ? (T)-tm.w : (T)tm.w;
+        switch (tm.ny & 3) {
+          case 0: cr += w; break;
+          case 1: ci += w; break;
+          case 2: cr -= w; break;
+          default: ci -= w; break;
+        }
+      }
+      hr += cr * q.x - ci * q.y;
+      hi += cr * q.y + ci * q.x;
+    }
+    if constexpr (ENERGY) {
+      er += (double)p.x * hr + (double)p.y * hi;
+      ei += (double)p.x * hi - (double)p.y * hr;
+    }
+    if constexpr (APPLY) {
+      const uint64_t off = b * sstride + base + ro[a >> L] + (a & lmask);
+      V2<T> o = lam_in ? lam_in[off] : mk<T>(0, 0);
+      lam_out[off] = mk<T>((T)(o.x + al * hr), (T)(o.y + al * hi));
+    }
+  }
+  if constexpr (ENERGY) {
+    block_sum2(er, ei);
+    if (threadIdx.x == 0) {
+      epart[b * estride + 2 * (P->slot0 + blk)] = er;
+      epart[b * estride + 2 * (P->slot0 + blk) + 1] = ei;
+    }
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_inner(const V2<T>* __restrict__ a, const V2<T>* __restrict__ c, int n,
                                                int64_t sstride, double* __restrict__ part, int64_t pstride) {
@@ -1123,8 +1229,8 @@ void launch_plain_gate(const DOp& op, const int32_t* ukbits_dev, const void* mta
 template <typename T>
 static size_t fused_lds_fwd(int k) { return sizeof(V2<T>) << k; }
 template <typename T>
-static size_t fused_lds_adj(int k, int ngop, int nw) {
-  return (2 * sizeof(V2<T>) << k) + sizeof(double) * (size_t)(ngop > 0 ? ngop : 1) * nw;
+static size_t fused_lds_adj(int k, int ngop, int nthr) {
+  return (2 * sizeof(V2<T>) << k) + sizeof(double) * (size_t)(ngop > 0 ? ngop : 1) * (nthr / 64);
 }
 
 // kernels that need more than 64 KiB of dynamic LDS must opt in once
@@ -1140,14 +1246,14 @@ static void ensure_lds_attr(K kern, size_t bytes) {
   d = bytes;
 }
 
-template <typename T, int R, int TB>
+template <typename T, int R, int TB, bool FULL>
 static void launch_fused_fwd_tb(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
                                 void* state, int64_t batch, int n, void* ckout, hipStream_t st) {
   const DPass& P = S.dpass[pi];
   const size_t lds = fused_lds_fwd<T>(P.k);
-  ensure_lds_attr(k_fused_fwd<T, R, TB>, lds);
+  ensure_lds_attr(k_fused_fwd<T, R, TB, FULL>, lds);
   const int64_t blocks = batch << P.nblk_log;
-  hipLaunchKernelGGL((k_fused_fwd<T, R, TB>), dim3((unsigned)blocks), dim3(1u << P.tthreads_log), lds, st,
+  hipLaunchKernelGGL((k_fused_fwd<T, R, TB, FULL>), dim3((unsigned)blocks), dim3(1u << P.tthreads_log), lds, st,
                      (const DPass*)D.dpass, pi, (const DGroup*)D.dgroup, (const DOp*)D.dop,
                      (const uint32_t*)D.runoff, (const T*)mtab, (int64_t)S.mtab_size, (const T*)ctab,
                      (V2<T>*)state, (int64_t)1 << n, (V2<T>*)ckout);
@@ -1157,21 +1263,23 @@ template <typename T, int R>
 void launch_fused_fwd(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
                       void* state, int64_t batch, int n, void* ckout, hipStream_t st) {
   const int thr = 1 << S.dpass[pi].tthreads_log;
-  if (thr <= 256) launch_fused_fwd_tb<T, R, 256>(S, D, pi, mtab, ctab, state, batch, n, ckout, st);
-  else if (thr <= 512) launch_fused_fwd_tb<T, R, 512>(S, D, pi, mtab, ctab, state, batch, n, ckout, st);
-  else launch_fused_fwd_tb<T, R, 1024>(S, D, pi, mtab, ctab, state, batch, n, ckout, st);
+  const bool full = S.dpass[pi].full != 0;
+#define TCM_FWD(TB, F) launch_fused_fwd_tb<T, R, TB, F>(S, D, pi, mtab, ctab, state, batch, n, ckout, st)
+  if (thr <= 256) { if (full) TCM_FWD(256, true); else TCM_FWD(256, false); }
+  else if (thr <= 512) { if (full) TCM_FWD(512, true); else TCM_FWD(512, false); }
+  else { if (full) TCM_FWD(1024, true); else TCM_FWD(1024, false); }
+#undef TCM_FWD
 }
 
-template <typename T, int R, int TB>
+template <typename T, int R, int TB, bool FULL>
 static void launch_fused_adj_tb(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
                                 const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch,
                                 int n, double* part, int64_t pstride, hipStream_t st) {
   const DPass& P = S.dpass[pi];
-  const int nw = (1 << P.tthreads_log) >> 6;
-  const size_t lds = fused_lds_adj<T>(P.k, P.ngop, nw);
-  ensure_lds_attr(k_fused_adj<T, R, TB>, lds);
+  const size_t lds = fused_lds_adj<T>(P.k, P.ngop, 1 << P.tthreads_log);
+  ensure_lds_attr(k_fused_adj<T, R, TB, FULL>, lds);
   const int64_t blocks = batch << P.nblk_log;
-  hipLaunchKernelGGL((k_fused_adj<T, R, TB>), dim3((unsigned)blocks), dim3(1u << P.tthreads_log), lds, st,
+  hipLaunchKernelGGL((k_fused_adj<T, R, TB, FULL>), dim3((unsigned)blocks), dim3(1u << P.tthreads_log), lds, st,
                      (const DPass*)D.dpass, pi, (const DGroup*)D.dgroup, (const DOp*)D.dop,
                      (const DMember*)D.dmem, (const uint32_t*)D.runoff, (const T*)mtab, (int64_t)S.mtab_size,
                      (const T*)ctab, (const V2<T>*)psi_in, (V2<T>*)psi_out, (const V2<T>*)lam_in, (V2<T>*)lam_out,
@@ -1183,12 +1291,13 @@ void launch_fused_adj(const Sched& S, const DeviceTables& D, int pi, const void*
                       const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n,
                       double* part, int64_t pstride, hipStream_t st) {
   const int thr = 1 << S.dpass[pi].tthreads_log;
-  if (thr <= 256)
-    launch_fused_adj_tb<T, R, 256>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st);
-  else if (thr <= 512)
-    launch_fused_adj_tb<T, R, 512>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st);
-  else
-    launch_fused_adj_tb<T, R, 1024>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st);
+  const bool full = S.dpass[pi].full != 0;
+#define TCM_ADJ(TB, F) \
+  launch_fused_adj_tb<T, R, TB, F>(S, D, pi, mtab, ctab, psi_in, psi_out, lam_in, lam_out, batch, n, part, pstride, st)
+  if (thr <= 256) { if (full) TCM_ADJ(256, true); else TCM_ADJ(256, false); }
+  else if (thr <= 512) { if (full) TCM_ADJ(512, true); else TCM_ADJ(512, false); }
+  else { if (full) TCM_ADJ(1024, true); else TCM_ADJ(1024, false); }
+#undef TCM_ADJ
 }
 
 template <typename T>
@@ -1228,6 +1337,29 @@ void launch_pauli(const DPGroup& G, const DPTerm* terms_dev, int n, const void* 
     hipLaunchKernelGGL((k_pauli<T, true, false>), grid, dim3(256), 0, st, G, terms_dev, n, p, ss, o, first, alpha, epart, estride);
   else
     hipLaunchKernelGGL((k_pauli<T, false, true>), grid, dim3(256), 0, st, G, terms_dev, n, p, ss, o, first, alpha, epart, estride);
+}
+
+template <typename T>
+void launch_pauli_tile(const DPPass& P, const DPPass* passes_dev, int pi, const DPGroupT* groups_dev,
+                       const DPTermT* terms_dev, const uint32_t* runoff_dev, const void* psi, const void* lam_in,
+                       void* lam_out, int n, const double* alpha, double* epart, int64_t estride, int64_t batch,
+                       bool energy, bool apply, hipStream_t st) {
+  const int64_t blocks = batch << P.nblk_log;
+  const size_t lds = sizeof(V2<T>) << P.k;
+  const dim3 grid((unsigned)blocks), block(1u << P.nthr_log);
+  const V2<T>* ps = (const V2<T>*)psi;
+  const V2<T>* li = (const V2<T>*)lam_in;
+  V2<T>* lo = (V2<T>*)lam_out;
+  const int64_t ss = 1ll << n;
+  if (energy && apply)
+    hipLaunchKernelGGL((k_pauli_tile<T, true, true>), grid, block, lds, st, passes_dev, pi, groups_dev, terms_dev,
+                       runoff_dev, ps, li, lo, ss, alpha, epart, estride);
+  else if (energy)
+    hipLaunchKernelGGL((k_pauli_tile<T, true, false>), grid, block, lds, st, passes_dev, pi, groups_dev, terms_dev,
+                       runoff_dev, ps, li, lo, ss, alpha, epart, estride);
+  else
+    hipLaunchKernelGGL((k_pauli_tile<T, false, true>), grid, block, lds, st, passes_dev, pi, groups_dev, terms_dev,
+                       runoff_dev, ps, li, lo, ss, alpha, epart, estride);
 }
 
 template <typename T>
@@ -1278,7 +1410,10 @@ void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, co
                                        hipStream_t);                                                           \
   template void launch_pauli<T>(const DPGroup&, const DPTerm*, int, const void*, void*, int, const double*,   \
                                 double*, int64_t, int64_t, bool, bool, hipStream_t);                           \
-  template void launch_inner<T>(const void*, const void*, int, int64_t, int, double*, int64_t, hipStream_t);
+  template void launch_inner<T>(const void*, const void*, int, int64_t, int, double*, int64_t, hipStream_t); \
+  template void launch_pauli_tile<T>(const DPPass&, const DPPass*, int, const DPGroupT*, const DPTermT*,        \
+                                     const uint32_t*, const void*, const void*, void*, int, const double*,       \
+                                     double*, int64_t, int64_t, bool, bool, hipStream_t);
 
 TCM_INST(float)
 TCM_INST(double)

@@ -648,6 +648,8 @@ static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant) {
         }
       }
       build_group_ops(P, S, hg, local_of_bit, gloc_of_gop, dg);
+      for (int o = dg.op0; o < dg.op0 + dg.nops; ++o)
+        if (S.dop[o].kind == OK_U2 || S.dop[o].gen == GN_M2) dp.full = 1;
       S.dgroup.push_back(dg);
     }
     S.dpass.push_back(dp);
@@ -701,8 +703,10 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
     const int Lw = dt == 0 ? 4 : 3;
     // default tiles keep one array at 64 KiB of LDS (forward) and two
     // arrays at 64 KiB together (adjoint)
-    int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 13 : 12);
-    int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (dt == 0 ? 12 : 11);
+    // defaults measured on MI355X: 32 KiB forward tiles (several blocks per
+    // CU overlap HBM and compute); the adjoint holds two tiles
+    int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 12 : 11);
+    int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (dt == 0 ? 11 : 10);
     // fused passes need at least one full wave (64 threads) per tile
     bool fused = o.fuse != 0 && n >= R + 6;
     for (int mode = 0; mode < 2; ++mode) {

@@ -91,6 +91,7 @@ struct DPass {
   int32_t ngop;       // gradient ops in this pass
   int32_t gop_list0;  // first entry of this pass in Sched::gop_list
   int32_t plain_op;   // >= 0: not a tile pass but one plain gate (Sched::dop index)
+  int32_t full;       // 1: the pass holds dense 4x4 ops / generators (full kernel)
   int32_t blk_bits[32];  // global bit positions of the block bits (ascending)
 };
 

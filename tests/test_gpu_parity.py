@@ -452,6 +452,29 @@ def test_pauli_sums_vs_oracle(gpu):
             np.testing.assert_allclose(hs, sv.pauli_sum_apply(s, n, terms), atol=tol * 10 * np.abs(w).sum())
 
 
+def test_pauli_wide_terms_and_tiles(gpu):
+    """Terms wider than an LDS tile (X on every qubit) run beside tiled ones."""
+    from tensorcircuit_amd import engine
+
+    rng = np.random.default_rng(5)
+    n = 14
+    s = rng.standard_normal(2**n) + 1j * rng.standard_normal(2**n)
+    s /= np.linalg.norm(s)
+    codes = [[1] * n, [2] * n, [3] * n, [2, 1] + [0] * (n - 2), [0] * (n - 1) + [1]]
+    codes += [list(rng.integers(0, 4, n)) for _ in range(6)]
+    w = rng.standard_normal(len(codes))
+    terms = [(float(w[k]), list(codes[k])) for k in range(len(codes))]
+    ham = engine.PauliSum(n, codes, w)
+    for dt, tol in ((torch.complex64, 1e-5), (torch.complex128, 1e-12)):
+        st = torch.as_tensor(s).to(dt).cuda()
+        e = np_of(ham.expect(st))[0]
+        ref = sv.pauli_sum_expectation(s, n, terms)
+        assert abs(e - ref) < tol * np.abs(w).sum() * 4, (dt, e, ref)
+        alpha = torch.tensor([0.5], dtype=torch.float64, device="cuda")
+        hs = np_of(ham.apply(st, alpha))[0]
+        np.testing.assert_allclose(hs, 0.5 * sv.pauli_sum_apply(s, n, terms), atol=tol * 10 * np.abs(w).sum())
+
+
 def test_heisenberg_ring_energy(gpu):
     """tests/test_quantum.py:316-321: the 6-site ring ground energy -11.2111
     from <psi|H|psi> on the exact ground state."""

@@ -101,7 +101,8 @@ def test_schedule_is_a_valid_reordering(n, k, R, dtype, mode):
 
 
 def test_hea_schedule_shape():
-    """24-qubit HEA layer packs into about two tile passes per layer."""
+    """24-qubit HEA layers pack into a few tile passes per layer (20 of the 24
+    qubits lie above the 4 always-resident low bits; a 12-bit tile takes 8)."""
     from tensorcircuit_amd import Circuit
     from tensorcircuit_amd.templates.blocks import hea_layers
 
@@ -114,7 +115,7 @@ def test_hea_schedule_shape():
     prog = Program(n, specs, len(thetas), fuse=1, ckpt_every=2)
     d = prog.describe("complex64", 0)
     npass = len(d["sched"]["passes"])
-    assert npass <= 3 * L, npass
+    assert npass <= 4 * L, npass
     assert prog.nckpt == 1  # after layer 2 (the barrier after the last layer is not a checkpoint)
     da = prog.describe("complex64", 1)
     assert len(da["sched"]["ckpt_after"]) == 1
