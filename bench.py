@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--config", choices=["c4", "c2", "c5"], default="c4",
+                    help="c4 (default): 24q adjoint energy+grad; c2: 26q forward + energy; "
+                         "c5: 30q fused-MFMA brickwork with a complex128 check")
     ap.add_argument("--kernel-only", choices=["fwd", "adj", "copy"], default=None,
                     help="run only the fused forward passes, the adjoint sweep, or a state copy of the same "
                          "size (for counter profiles); prints no JSON")
@@ -57,6 +60,10 @@ def main():
     from tensorcircuit_amd import engine
     from tensorcircuit_amd.templates.blocks import hea_layers
 
+    if args.config == "c5":
+        return run_c5(args)
+    if args.config == "c2":
+        return run_c2(args)
     n, L, B = args.n, args.layers, args.batch
     dev = engine.device()
     c = tc.Circuit(n)
@@ -206,6 +213,99 @@ def main():
         print("cpu baseline: %.1f s per eval (sample %.1f s, %d threads)" % (cpu_eval_s, dt, threads),
               file=sys.stderr)
 
+    print(json.dumps(out))
+
+
+def _timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def run_c5(args):
+    """30-qubit depth-20 brickwork (Haar SU(2) + CZ, seed 1005) fused into
+    5-qubit 32x32 blocks applied on FP32 matrix cores; the same blocks in
+    complex128 on the VALU serve as the on-GPU reference."""
+    import torch
+
+    from tensorcircuit_amd import engine, fusion, workloads
+
+    n, depth = 30, 20
+    t_stage = time.perf_counter()
+    gates = workloads.brickwork(n, depth, 1005)
+    blocks = fusion.fuse(gates, n)
+    p32 = engine.Program(n, blocks, 0, fuse=1, mfma=1)
+    staging = time.perf_counter() - t_stage
+    st32 = torch.empty(1, 1 << n, dtype=torch.complex64, device="cuda")
+    dt32, _ = _timed(lambda: p32.forward(None, 1, "complex64", out=st32), args.steps, args.warmup)
+    p64 = engine.Program(n, blocks, 0, fuse=1, mfma=0)
+    st64 = torch.empty(1, 1 << n, dtype=torch.complex128, device="cuda")
+    dt64, _ = _timed(lambda: p64.forward(None, 1, "complex128", out=st64), 1, 0)
+    fid = abs(complex(engine.inner(st64, st32.to(torch.complex128), n)[0])) ** 2
+    del st64
+    nb = len(blocks)
+    flops = nb * 8.0 * 32 * (1 << n)          # 32 complex MACs per amplitude per block
+    bytes_ = nb * 2 * 8.0 * (1 << n)          # read + write of the c64 state per block
+    out = {"metric": "C5 fused-MFMA brickwork states/s (30q, depth 20)", "value": round(1.0 / dt32, 4),
+           "unit": "states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(dt32 * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64", "data": "synthetic",
+           "config": {"workload": "brickwork_haar_su2_cz", "nqubits": n, "depth": depth, "gates": len(gates),
+                      "fused_blocks": nb, "seed": 1005, "staging_s": round(staging, 3)},
+           "fidelity_vs_c128": fid, "c128_ms": round(dt64 * 1e3, 3),
+           "mfma": {"achieved": round(flops / dt32 / 1e12, 2), "peak": 157.3, "unit": "TFLOP/s",
+                    "frac": round(flops / dt32 / 1e12 / 157.3, 4)},
+           "hbm": {"achieved": round(bytes_ / dt32 / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(bytes_ / dt32 / 1e9 / HBM_PEAK_GBS, 4)}}
+    print("c5: %.2f ms/state (%d blocks), c128 %.1f ms, fidelity %.9f" % (dt32 * 1e3, nb, dt64 * 1e3, fid),
+          file=sys.stderr)
+    print(json.dumps(out))
+
+
+def run_c2(args):
+    """26-qubit 10-layer HEA forward + TFIM energy (seed 1002) in complex64,
+    checked against the complex128 run of the same program."""
+    import numpy as np
+    import torch
+
+    import tensorcircuit_amd as tc
+    from tensorcircuit_amd import engine, workloads
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L = 26, 10
+    c = tc.Circuit(n)
+    hea_layers(c, np.zeros((2 * L, n)), L)
+    specs, thetas = c._lower()
+    codes, w = workloads.tfim_terms(n)
+    ham = engine.PauliSum(n, codes, w)
+    p = torch.as_tensor(workloads.hea_params(1002, L, n).reshape(1, -1)).cuda()
+    prog = engine.Program(n, specs, len(thetas), fuse=1)
+    st = torch.empty(1, 1 << n, dtype=torch.complex64, device="cuda")
+
+    def step():
+        prog.forward(p, 1, "complex64", out=st)
+        return ham.expect(st)
+
+    dt, e32 = _timed(step, args.steps, args.warmup)
+    st64, _ = prog.forward(p.double(), 1, "complex128")
+    e64 = ham.expect(st64)
+    fid = abs(complex(engine.inner(st64, st.to(torch.complex128), n)[0])) ** 2
+    out = {"metric": "C2 26q HEA forward + TFIM energy per s", "value": round(1.0 / dt, 3), "unit": "evals/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+           "data": "synthetic", "config": {"workload": "hea_tfim_forward", "nqubits": n, "layers": L,
+                                           "fwd_passes": prog.npasses, "seed": 1002},
+           "energy_c64": float(e32[0].real), "energy_c128": float(e64[0].real),
+           "abs_denergy": abs(float(e32[0].real) - float(e64[0].real)), "fidelity_vs_c128": fid}
+    print("c2: %.2f ms/eval, E=%.8f (c128 %.8f)" % (dt * 1e3, out["energy_c64"], out["energy_c128"]),
+          file=sys.stderr)
     print(json.dumps(out))
 
 

@@ -88,8 +88,10 @@ typedef struct tcm_opts {
   int32_t adj_tile_bits; /* adjoint tile size log2; 0 = auto */
   int32_t ckpt_every;  /* adjoint: checkpoint the forward state every
                           `ckpt_every` barriers (0 = none) */
-  int32_t reg_slots;   /* register-resident qubits per thread (3..5); 0 = auto */
-  int32_t reserved[3];
+  int32_t reg_slots;   /* register-resident qubits per thread; 0 = auto (4) */
+  int32_t mfma;        /* 1: dense 5-qubit gates run on FP32 matrix cores
+                          (complex64 only; complex128 keeps the VALU kernel) */
+  int32_t reserved[2];
 } tcm_opts;
 
 typedef struct tcm_program tcm_program; /* opaque, caller-owned */

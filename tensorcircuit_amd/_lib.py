@@ -42,7 +42,8 @@ class TcmOpts(C.Structure):
         ("adj_tile_bits", C.c_int32),
         ("ckpt_every", C.c_int32),
         ("reg_slots", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("mfma", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -934,6 +934,82 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
 }
 
 // ---------------------------------------------------------------------------
+// dense 5-qubit gate on FP32 matrix cores.  The state is viewed as a
+// 32 x (2^(n-5)) matrix X (rows = the 5 gate bits, row MSB = gate qubit 0,
+// columns = all other bits) and Y = U X is computed per 32-column chunk by
+// one wave with v_mfma_f32_32x32x2f32 (K = 2 per instruction):
+//   Yr = Ur Xr - Ui Xi,  Yi = Ur Xi + Ui Xr.
+// Operand layouts (CDNA3/4): lane l supplies A[l % 32][l / 32] and
+// B[l / 32][l % 32]; accumulator register r of lane l holds
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// ---------------------------------------------------------------------------
+typedef float tcm_v16f __attribute__((ext_vector_type(16)));
+
+template <bool DAG>
+__global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restrict__ ukbits,
+                                                 const float* __restrict__ ct, float2* __restrict__ st,
+                                                 int64_t sstride, int n, int64_t batch) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const float* U = ct + (-op.moff - 1);
+  int bits[5], sorted[5];
+  for (int i = 0; i < 5; ++i) sorted[i] = bits[i] = ukbits[op.mem0 + i];
+  for (int i = 0; i < 5; ++i)
+    for (int j = i + 1; j < 5; ++j)
+      if (sorted[j] < sorted[i]) { int t = sorted[i]; sorted[i] = sorted[j]; sorted[j] = t; }
+  // A operands for the 16 K-steps: U[i][2s + l/32] (U^dag: conj(U[2s + l/32][i]))
+  const int ai = lane & 31, ak = lane >> 5;
+  float ur[16], ui[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int kk = 2 * s + ak;
+    if (DAG) {
+      ur[s] = U[2 * (kk * 32 + ai)];
+      ui[s] = -U[2 * (kk * 32 + ai) + 1];
+    } else {
+      ur[s] = U[2 * (ai * 32 + kk)];
+      ui[s] = U[2 * (ai * 32 + kk) + 1];
+    }
+  }
+  // index offset of gate row r (5 bits, MSB = bits[0])
+  auto rowoff = [&](int r) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      if ((r >> (4 - i)) & 1) o |= 1ull << bits[i];
+    return o;
+  };
+  const int64_t ncol = 1ll << (n - 5);
+  const int64_t chunks_per_state = ncol >> 5;
+  const int64_t nchunks = chunks_per_state * batch;
+  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+    const int64_t b = ch / chunks_per_state;
+    const uint64_t c = ((uint64_t)(ch % chunks_per_state) << 5) + (lane & 31);
+    uint64_t base = c;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) base = ins0(base, sorted[i]);
+    float2* s = st + b * sstride + base;
+    float2 x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = s[rowoff(2 * q + ak)];
+    tcm_v16f dr = {0}, di = {0};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      dr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].x, dr, 0, 0, 0);
+      dr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[q], x[q].y, dr, 0, 0, 0);
+      di = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].y, di, 0, 0, 0);
+      di = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[q], x[q].x, di, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r >> 2) + 4 * ak + (r & 3);
+      s[rowoff(row)] = make_float2(dr[r], di[r]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
 // out[b * ncomp + c] = sum_s part[b * bstride + s * ncomp + c]
@@ -1216,6 +1292,15 @@ void launch_plain_gate(const DOp& op, const int32_t* ukbits_dev, const void* mta
                          ct, s, N, n, batch);
       break;
     case OK_UK:
+      if constexpr (std::is_same<T, float>::value) {
+        if (op.nmem == 5 && (op.flags & OP_F_MFMA) && n >= 10) {
+          const int64_t waves = batch * (N >> 10);
+          const int blocks = (int)std::min<int64_t>((waves + 3) / 4, 1 << 20);
+          hipLaunchKernelGGL((k_mfma_u5<DAG>), dim3(blocks), dim3(256), 0, st, op, ukbits_dev, ct, (float2*)s,
+                             N, n, batch);
+          break;
+        }
+      }
       switch (op.nmem) {
         case 3: hipLaunchKernelGGL((k_plaink<T, 3, DAG>), dim3(grid_for(batch * (N >> 3))), dim3(256), 0, st, op, ukbits_dev, ct, s, N, n, batch); break;
         case 4: hipLaunchKernelGGL((k_plaink<T, 4, DAG>), dim3(grid_for(batch * (N >> 4))), dim3(256), 0, st, op, ukbits_dev, ct, s, N, n, batch); break;

@@ -388,6 +388,7 @@ static void sched_plain(const Program& P, Sched& S) {
       op.nmem = g.nq;
       op.mem0 = (int)S.ukbits.size();
       for (int i = 0; i < g.nq; ++i) S.ukbits.push_back(g.bits[i]);
+      if (g.nq == 5 && P.opts.mfma) op.flags |= OP_F_MFMA;
     }
     S.dop.push_back(op);
   }
@@ -593,6 +594,7 @@ static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant) {
       op.nmem = g.nq;
       op.mem0 = (int)S.ukbits.size();
       for (int i = 0; i < g.nq; ++i) S.ukbits.push_back(g.bits[i]);
+      if (g.nq == 5 && P.opts.mfma) op.flags |= OP_F_MFMA;
       dp.plain_op = (int)S.dop.size();
       S.dop.push_back(op);
       dp.gop_list0 = (int)S.gop_list.size();

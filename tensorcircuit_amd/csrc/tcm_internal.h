@@ -61,7 +61,11 @@ struct DOp {
   int32_t mem0;        // OK_DG: first entry in the member table; OK_UK: first
                        // entry in the bit table
   int32_t gloc;        // index of `gop` among the gradient ops of its pass
+  int32_t flags;       // OP_F_* bits
 };
+
+// DOp flags
+enum : int32_t { OP_F_MFMA = 1 };  // dense 5-qubit op applied with FP32 MFMA
 
 // Gradient member of an OK_DG op (diagonal generator on slots of the group).
 struct DMember {

@@ -494,3 +494,55 @@ def test_heisenberg_ring_energy(gpu):
     ham = engine.PauliSum(n, codes, w)
     e = np_of(ham.expect(torch.as_tensor(evecs[:, 0]).to(torch.complex128).cuda()))[0]
     np.testing.assert_allclose(e.real, -11.2111, atol=1e-4)
+
+
+# ----------------------------------------------------- MFMA 5-qubit blocks ---
+
+@pytest.mark.parametrize("n", [10, 12, 15])
+def test_mfma_dense5_vs_oracle(gpu, n):
+    """Dense 5-qubit gates on FP32 matrix cores (every placement class: gate
+    bits among the low bits, the high bits and mixed) vs the c128 oracle, and
+    the same gates undone through the adjoint (U^dagger on the matrix cores)."""
+    from tensorcircuit_amd import _lib, engine
+    from tensorcircuit_amd.engine import GateSpec
+
+    rng = np.random.default_rng(n)
+    specs, ops = [], []
+    placements = [tuple(range(5)), tuple(range(n - 5, n)), (0, n - 1, 2, n // 2, n - 3)]
+    for k in range(6):
+        qs = placements[k % 3] if k < 3 else tuple(int(q) for q in rng.permutation(n)[:5])
+        u = np.linalg.qr(rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32)))[0]
+        specs.append(GateSpec(_lib.G_MAT, qs, -1, 0.0, 1.0, u))
+        ops.append(sv.Op("unitary", qs, kw={"unitary": u}))
+    s0 = rng.standard_normal(2**n) + 1j * rng.standard_normal(2**n)
+    s0 /= np.linalg.norm(s0)
+    ref = sv.run_ops(n, ops, inputs=s0)
+    for mfma in (1, 0):
+        prog = engine.Program(n, specs, 0, fuse=1, mfma=mfma)
+        st, _ = prog.forward(None, 2, "complex64", inputs=torch.as_tensor(np.stack([s0, s0])).to(torch.complex64))
+        np.testing.assert_allclose(np_of(st)[1], ref, atol=2e-6)
+        # the vector-Jacobian product with grad = final state maps back to U^dag psi = s0
+        _, gi = prog.vjp(None, None, st, st, 2, "complex64", True)
+        np.testing.assert_allclose(np_of(gi)[0], s0, atol=2e-6)
+
+
+def test_brickwork_fused_mfma_vs_c128(gpu):
+    """C5-shaped check at 16 qubits: fused 5-qubit blocks on FP32 MFMA vs the
+    complex128 VALU run of the same blocks and vs the c128 oracle at 14 qubits."""
+    from tensorcircuit_amd import engine, fusion, workloads
+
+    for n, depth in ((14, 8), (16, 10)):
+        gates = workloads.brickwork(n, depth, 1005)
+        blocks = fusion.fuse(gates, n)
+        p32 = engine.Program(n, blocks, 0, fuse=1, mfma=1)
+        p64 = engine.Program(n, blocks, 0, fuse=1, mfma=0)
+        s32, _ = p32.forward(None, 1, "complex64")
+        s64, _ = p64.forward(None, 1, "complex128")
+        fid = abs(complex(engine.inner(s64, s32.to(torch.complex128), n)[0])) ** 2
+        assert abs(fid - 1.0) < 1e-5, fid
+        if n == 14:
+            ref = sv.zero_state(n)
+            for g in gates:
+                ref = sv.apply_gate(ref, g.mat, g.qubits, n)
+            np.testing.assert_allclose(np_of(s64).reshape(-1), ref, atol=1e-12)
+            np.testing.assert_allclose(np_of(s32).reshape(-1), ref, atol=2e-6)

@@ -191,3 +191,34 @@ def test_circuit_records_reference_qir():
         c.cnot(1, 1)
     with pytest.raises(ValueError):
         c.expectation_ps(x=[0], z=[0])
+
+
+def test_fusion_matches_unfused_oracle():
+    """fuse() multiplies gates into <=5-qubit dense blocks; replaying the
+    blocks through the oracle reproduces the unfused circuit."""
+    from tensorcircuit_amd import fusion, workloads
+
+    n = 9
+    gates = workloads.brickwork(n, 7, 11)
+    blocks = fusion.fuse(gates, n)
+    assert all(len(b.qubits) == 5 for b in blocks)
+    assert len(blocks) < len(gates) // 4
+    s1 = sv.zero_state(n)
+    for g in gates:
+        s1 = sv.apply_gate(s1, g.mat, g.qubits, n)
+    s2 = sv.zero_state(n)
+    for b in blocks:
+        np.testing.assert_allclose(b.mat.conj().T @ b.mat, np.eye(32), atol=1e-12)
+        s2 = sv.apply_gate(s2, b.mat, b.qubits, n)
+    np.testing.assert_allclose(s2, s1, atol=1e-12)
+
+
+def test_brickwork_generator_matches_oracle():
+    from tensorcircuit_amd import workloads
+
+    o = sv.brickwork_ops(12, 3, 1005)
+    w = workloads.brickwork(12, 3, 1005)
+    assert len(o) == len(w)
+    for a, b in zip(o, w):
+        assert tuple(a.qubits) == tuple(b.qubits)
+        np.testing.assert_array_equal(a.matrix(), b.mat)
