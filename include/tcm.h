@@ -207,6 +207,26 @@ int tcm_adjoint_vjp(tcm_program* prog, int32_t dtype, const void* params,
                     void* grad_input, void* workspace, int64_t ws_bytes,
                     void* stream);
 
+/* ---- batched uneven circuits ----------------------------------------------
+ * Many circuits of different programs / qubit counts in few launches (the
+ * reference's vmap over circuit structures, backends/pytorch_backend.py:
+ * 605-617, and chunked adaptive_vmap, experimental.py:17-74): descriptor i
+ * runs its program on `batch` states at `state` with parameters at `params`
+ * (param_stride apart).  Pass r of every program goes into one launch. */
+typedef struct tcm_multi_desc {
+  tcm_program* prog;
+  const void* params;
+  int64_t param_stride;
+  void* state;
+  int64_t batch;
+} tcm_multi_desc;
+
+int tcm_multi_workspace_bytes(const tcm_multi_desc* descs, int32_t ndesc,
+                              int32_t dtype, int64_t* bytes);
+int tcm_forward_multi(const tcm_multi_desc* descs, int32_t ndesc,
+                      int32_t dtype, int32_t init_zero, void* workspace,
+                      int64_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,6 @@
 """Build libtcm.so (HIP kernels for gfx950 + the C ABI) in-tree.
 
-    python -m tensorcircuit_amd._build
+    python tensorcircuit_amd/_build.py
 
 Objects go to ``tensorcircuit_amd/_build/``; the library is written to
 ``tensorcircuit_amd/libtcm.so``.
@@ -35,7 +35,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
     else:
-        cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + COMMON + ["--offload-arch=" + ARCH, "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

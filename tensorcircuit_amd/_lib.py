@@ -71,7 +71,7 @@ class TcmMultiDesc(C.Structure):
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        "libtcm.so not built (expected at %s); run `python -m tensorcircuit_amd._build`" % LIB_PATH)
+        "libtcm.so not built (expected at %s); run `python tensorcircuit_amd/_build.py`" % LIB_PATH)
 
 lib = C.CDLL(LIB_PATH)
 
@@ -97,6 +97,8 @@ SIGNATURES = {
     "tcm_adjoint_workspace_bytes": (C.c_int, [_vp, _vp, _i32, _i64, _i32, C.POINTER(_i64)]),
     "tcm_adjoint_energy_grad": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "tcm_adjoint_vjp": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
+    "tcm_multi_workspace_bytes": (C.c_int, [C.POINTER(TcmMultiDesc), _i32, _i32, C.POINTER(_i64)]),
+    "tcm_forward_multi": (C.c_int, [C.POINTER(TcmMultiDesc), _i32, _i32, _i32, _vp, _i64, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

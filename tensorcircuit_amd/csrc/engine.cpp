@@ -538,6 +538,78 @@ int vjp_impl(Program& P, int dtype, const void* params, int64_t pstride, const v
 
 bool valid_dtype(int d) { return d == TCM_C64 || d == TCM_C128; }
 
+// ---- batched uneven circuits ------------------------------------------------
+int64_t multi_ws(const tcm_multi_desc* descs, int ndesc, int dtype) {
+  int64_t total = 0;
+  for (int i = 0; i < ndesc; ++i)
+    total += align_up(descs[i].batch * (int64_t)std::max(descs[i].prog->p.sched[dtype][0].mtab_size, 1) *
+                      (int64_t)tsize(dtype));
+  return total;
+}
+
+template <typename T>
+int forward_multi_impl(const tcm_multi_desc* descs, int ndesc, int dtype, int init_zero, void* ws, hipStream_t st) {
+  std::vector<DeviceTables*> D(ndesc);
+  std::vector<void*> mtab(ndesc);
+  Carve cv(ws);
+  int maxp = 0;
+  for (int i = 0; i < ndesc; ++i) {
+    Program& P = descs[i].prog->p;
+    if (int rc = ensure_tables(P, dtype, 0, &D[i])) return rc;
+    const Sched& S = P.sched[dtype][0];
+    mtab[i] = cv.take(descs[i].batch * (int64_t)std::max(S.mtab_size, 1) * sizeof(T));
+    if (descs[i].batch <= 0) continue;
+    launch_prep<T>(S, *D[i], D[i]->ctab, descs[i].params, descs[i].param_stride, mtab[i], descs[i].batch, st);
+    if (init_zero) launch_init_zero<T>(descs[i].state, P.n, descs[i].batch, st);
+    maxp = std::max(maxp, (int)S.passes.size());
+  }
+  for (int r = 0; r < maxp; ++r) {
+    MultiJobs J{};
+    int64_t blocks = 0;
+    int threads = 64;
+    size_t lds = 0;
+    auto flush = [&]() {
+      if (J.njobs == 0) return;
+      launch_fused_fwd_multi<T>(J, blocks, threads, lds, st);
+      J = MultiJobs{};
+      blocks = 0;
+      threads = 64;
+      lds = 0;
+    };
+    for (int i = 0; i < ndesc; ++i) {
+      const Program& P = descs[i].prog->p;
+      const Sched& S = P.sched[dtype][0];
+      if (descs[i].batch <= 0 || r >= (int)S.passes.size()) continue;
+      if (S.k == 0 || S.dpass[r].plain_op >= 0) {
+        const DOp& op = S.k > 0 ? S.dop[S.dpass[r].plain_op] : S.dop[r];
+        launch_plain_gate<T, false>(op, (const int32_t*)D[i]->ukbits, mtab[i], S.mtab_size, D[i]->ctab,
+                                    descs[i].state, P.n, descs[i].batch, st);
+        continue;
+      }
+      if (S.R != 4) return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
+      const DPass& dp = S.dpass[r];
+      MultiJob& q = J.job[J.njobs++];
+      q.dpass = D[i]->dpass;
+      q.dgroup = D[i]->dgroup;
+      q.dop = D[i]->dop;
+      q.runoff = D[i]->runoff;
+      q.ctab = D[i]->ctab;
+      q.mtab = mtab[i];
+      q.state = descs[i].state;
+      q.mstride = S.mtab_size;
+      q.sstride = 1ll << P.n;
+      q.block0 = blocks;
+      q.pass = r;
+      blocks += descs[i].batch << dp.nblk_log;
+      threads = std::max(threads, 1 << dp.tthreads_log);
+      lds = std::max(lds, sizeof(V2<T>) << dp.k);
+      if (J.njobs == kMaxMultiJobs) flush();
+    }
+    flush();
+  }
+  return 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -784,6 +856,31 @@ int tcm_adjoint_energy_grad(tcm_program* prog, int32_t dtype, const void* params
   return dtype == TCM_C64
              ? energy_grad_impl<float>(prog->p, dtype, params, param_stride, H, inputs, batch, energy, grad, workspace, ws_bytes, st)
              : energy_grad_impl<double>(prog->p, dtype, params, param_stride, H, inputs, batch, energy, grad, workspace, ws_bytes, st);
+}
+
+int tcm_multi_workspace_bytes(const tcm_multi_desc* descs, int32_t ndesc, int32_t dtype, int64_t* bytes) {
+  if (!bytes || !valid_dtype(dtype) || ndesc < 0 || (ndesc > 0 && !descs)) return set_error(TCM_E_ARG, "bad arguments");
+  for (int i = 0; i < ndesc; ++i)
+    if (!descs[i].prog || descs[i].batch < 0) return set_error(TCM_E_ARG, "descriptor %d: bad program or batch", i);
+  *bytes = multi_ws(descs, ndesc, dtype);
+  return 0;
+}
+
+int tcm_forward_multi(const tcm_multi_desc* descs, int32_t ndesc, int32_t dtype, int32_t init_zero, void* workspace,
+                      int64_t ws_bytes, void* stream) {
+  if (!valid_dtype(dtype) || ndesc < 0 || (ndesc > 0 && !descs)) return set_error(TCM_E_ARG, "bad arguments");
+  for (int i = 0; i < ndesc; ++i) {
+    const tcm_multi_desc& d = descs[i];
+    if (!d.prog || d.batch < 0 || (d.batch > 0 && !d.state)) return set_error(TCM_E_ARG, "descriptor %d: bad arguments", i);
+    if (d.prog->p.nparams > 0 && d.batch > 0 && !d.params) return set_error(TCM_E_ARG, "descriptor %d: params required", i);
+  }
+  if (multi_ws(descs, ndesc, dtype) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = dtype == TCM_C64 ? forward_multi_impl<float>(descs, ndesc, dtype, init_zero, workspace, st)
+                            : forward_multi_impl<double>(descs, ndesc, dtype, init_zero, workspace, st);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError(), "forward_multi launch");
+  return 0;
 }
 
 int tcm_adjoint_vjp(tcm_program* prog, int32_t dtype, const void* params, int64_t param_stride, const void* ckpts,

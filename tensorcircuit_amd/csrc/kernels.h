@@ -55,6 +55,20 @@ struct DPTermT {
   double w;
 };
 
+// one program's pass inside a multi-program launch (passed by value)
+struct MultiJob {
+  const void *dpass, *dgroup, *dop, *runoff, *ctab, *mtab;
+  void* state;
+  int64_t mstride, sstride;
+  int64_t block0;   // first block of this job in the launch
+  int32_t pass, pad;
+};
+constexpr int kMaxMultiJobs = 24;
+struct MultiJobs {
+  int32_t njobs, pad;
+  MultiJob job[kMaxMultiJobs];
+};
+
 // device copies of one schedule's descriptors (owned by the program)
 struct DeviceTables {
   void* dpass = nullptr;
@@ -106,6 +120,8 @@ void launch_pauli_tile(const DPPass& P, const DPPass* passes_dev, int pi, const 
                        const DPTermT* terms_dev, const uint32_t* runoff_dev, const void* psi, const void* lam_in,
                        void* lam_out, int n, const double* alpha, double* epart, int64_t estride, int64_t batch,
                        bool energy, bool apply, hipStream_t st);
+template <typename T>
+void launch_fused_fwd_multi(const MultiJobs& J, int64_t blocks, int threads, size_t lds, hipStream_t st);
 template <typename T>
 void launch_inner(const void* a, const void* c, int n, int64_t batch, int nblk, double* part, int64_t pstride,
                   hipStream_t st);

@@ -786,54 +786,94 @@ __device__ __forceinline__ uint64_t block_base(const DPass* P, uint64_t blk) {
   return base;
 }
 
-template <typename T, int R, int TB, bool FULL>
-__global__ void __launch_bounds__(TB) k_fused_fwd(const DPass* __restrict__ passes, int pi,
-                                                    const DGroup* __restrict__ groups, const DOp* __restrict__ ops,
-                                                    const uint32_t* __restrict__ runoff, const T* __restrict__ mtab,
-                                                    int64_t mstride, const T* __restrict__ ct,
-                                                    V2<T>* __restrict__ st, int64_t sstride,
-                                                    V2<T>* __restrict__ ckout) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  V2<T>* tile = reinterpret_cast<V2<T>*>(smem);
-  const DPass* P = passes + pi;
-  const int k = P->k, L = P->L, nbl = P->nblk_log;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
-  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
-  const uint64_t base = block_base(P, blk);
-  V2<T>* s = st + b * sstride + base;
-  const T* mt = mtab + b * mstride;
+// Body of one fused forward pass over tile `blk` of batch item `b`; threads
+// tid >= nthr (a smaller tile than the block) only take part in barriers.
+template <typename T, int R, bool FULL>
+__device__ __forceinline__ void fused_fwd_tile(const DPass* P, const DGroup* __restrict__ groups,
+                                               const DOp* __restrict__ ops, const uint32_t* __restrict__ runoff,
+                                               const T* mt, const T* __restrict__ ct, V2<T>* __restrict__ s,
+                                               V2<T>* __restrict__ c, int nthr, V2<T>* tile) {
+  const int k = P->k, L = P->L;
+  const int tid = threadIdx.x;
+  const bool act = tid < nthr;
   const uint32_t* ro = runoff + P->runoff0;
   const uint32_t lmask = (1u << L) - 1;
+  if (act) {
 #pragma unroll
-  for (int it = 0; it < (1 << R); ++it) {
-    const uint32_t a = it * nthr + tid;
-    tile[swz(a)] = s[ro[a >> L] + (a & lmask)];
+    for (int it = 0; it < (1 << R); ++it) {
+      const uint32_t a = it * nthr + tid;
+      tile[swz(a)] = s[ro[a >> L] + (a & lmask)];
+    }
   }
   __syncthreads();
   for (int g = 0; g < P->ngroups; ++g) {
     const DGroup* G = groups + P->group0 + g;
-    uint32_t tb = 0;
-    for (int i = 0; i < k - R; ++i)
-      if ((tid >> i) & 1) tb |= 1u << G->cb[i];
-    const uint32_t a0 = swz(tb);
-    V2<T> v[1 << R];
+    if (act) {
+      uint32_t tb = 0;
+      for (int i = 0; i < k - R; ++i)
+        if ((tid >> i) & 1) tb |= 1u << G->cb[i];
+      const uint32_t a0 = swz(tb);
+      V2<T> v[1 << R];
 #pragma unroll
-    for (int j = 0; j < (1 << R); ++j) v[j] = tile[a0 ^ (uint32_t)G->S[j]];
-    for (int o = 0; o < G->nops; ++o) apply_op<T, R, false, FULL>(v, ops[G->op0 + o], mt, ct);
+      for (int j = 0; j < (1 << R); ++j) v[j] = tile[a0 ^ (uint32_t)G->S[j]];
+      for (int o = 0; o < G->nops; ++o) apply_op<T, R, false, FULL>(v, ops[G->op0 + o], mt, ct);
 #pragma unroll
-    for (int j = 0; j < (1 << R); ++j) tile[a0 ^ (uint32_t)G->S[j]] = v[j];
+      for (int j = 0; j < (1 << R); ++j) tile[a0 ^ (uint32_t)G->S[j]] = v[j];
+    }
     __syncthreads();
   }
-  V2<T>* c = ckout ? ckout + b * sstride + base : nullptr;
+  if (act) {
 #pragma unroll
-  for (int it = 0; it < (1 << R); ++it) {
-    const uint32_t a = it * nthr + tid;
-    const V2<T> x = tile[swz(a)];
-    const uint32_t off = ro[a >> L] + (a & lmask);
-    s[off] = x;
-    if (c) c[off] = x;
+    for (int it = 0; it < (1 << R); ++it) {
+      const uint32_t a = it * nthr + tid;
+      const V2<T> x = tile[swz(a)];
+      const uint32_t off = ro[a >> L] + (a & lmask);
+      s[off] = x;
+      if (c) c[off] = x;
+    }
   }
+}
+
+template <typename T, int R, int TB, bool FULL>
+__global__ void __launch_bounds__(TB) k_fused_fwd(const DPass* __restrict__ passes, int pi,
+                                                  const DGroup* __restrict__ groups, const DOp* __restrict__ ops,
+                                                  const uint32_t* __restrict__ runoff, const T* __restrict__ mtab,
+                                                  int64_t mstride, const T* __restrict__ ct,
+                                                  V2<T>* __restrict__ st, int64_t sstride,
+                                                  V2<T>* __restrict__ ckout) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const DPass* P = passes + pi;
+  const int nbl = P->nblk_log;
+  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
+  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
+  const uint64_t base = block_base(P, blk);
+  fused_fwd_tile<T, R, FULL>(P, groups, ops, runoff, mtab + b * mstride, ct, st + b * sstride + base,
+                             ckout ? ckout + b * sstride + base : nullptr, blockDim.x,
+                             reinterpret_cast<V2<T>*>(smem));
+}
+
+// Several programs' passes in one launch (circuits of different qubit
+// counts): block -> job by the prefix of the jobs' block counts.
+template <typename T, int R, int TB>
+__global__ void __launch_bounds__(TB) k_fused_fwd_multi(MultiJobs J) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int j = 0;
+  while (j + 1 < J.njobs && (int64_t)blockIdx.x >= J.job[j + 1].block0) ++j;
+  const MultiJob& Q = J.job[j];
+  const DPass* P = (const DPass*)Q.dpass + Q.pass;
+  const int nbl = P->nblk_log;
+  const uint64_t lb = (uint64_t)blockIdx.x - Q.block0;
+  const uint64_t blk = lb & ((1ull << nbl) - 1);
+  const uint64_t b = lb >> nbl;
+  const uint64_t base = block_base(P, blk);
+  const T* mt = (const T*)Q.mtab + b * Q.mstride;
+  V2<T>* s = (V2<T>*)Q.state + b * Q.sstride + base;
+  if (P->full)
+    fused_fwd_tile<T, R, true>(P, (const DGroup*)Q.dgroup, (const DOp*)Q.dop, (const uint32_t*)Q.runoff, mt,
+                               (const T*)Q.ctab, s, nullptr, 1 << P->tthreads_log, reinterpret_cast<V2<T>*>(smem));
+  else
+    fused_fwd_tile<T, R, false>(P, (const DGroup*)Q.dgroup, (const DOp*)Q.dop, (const uint32_t*)Q.runoff, mt,
+                                (const T*)Q.ctab, s, nullptr, 1 << P->tthreads_log, reinterpret_cast<V2<T>*>(smem));
 }
 
 // Reverse sweep over one fused pass.  psi/lambda tiles are read from
@@ -1356,6 +1396,17 @@ void launch_fused_fwd(const Sched& S, const DeviceTables& D, int pi, const void*
 #undef TCM_FWD
 }
 
+template <typename T>
+void launch_fused_fwd_multi(const MultiJobs& J, int64_t blocks, int threads, size_t lds, hipStream_t st) {
+  if (threads <= 256) {
+    ensure_lds_attr(k_fused_fwd_multi<T, 4, 256>, lds);
+    hipLaunchKernelGGL((k_fused_fwd_multi<T, 4, 256>), dim3((unsigned)blocks), dim3(threads), lds, st, J);
+  } else {
+    ensure_lds_attr(k_fused_fwd_multi<T, 4, 512>, lds);
+    hipLaunchKernelGGL((k_fused_fwd_multi<T, 4, 512>), dim3((unsigned)blocks), dim3(threads), lds, st, J);
+  }
+}
+
 template <typename T, int R, int TB, bool FULL>
 static void launch_fused_adj_tb(const Sched& S, const DeviceTables& D, int pi, const void* mtab, const void* ctab,
                                 const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch,
@@ -1496,6 +1547,7 @@ void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, co
   template void launch_pauli<T>(const DPGroup&, const DPTerm*, int, const void*, void*, int, const double*,   \
                                 double*, int64_t, int64_t, bool, bool, hipStream_t);                           \
   template void launch_inner<T>(const void*, const void*, int, int64_t, int, double*, int64_t, hipStream_t); \
+  template void launch_fused_fwd_multi<T>(const MultiJobs&, int64_t, int, size_t, hipStream_t);                 \
   template void launch_pauli_tile<T>(const DPPass&, const DPPass*, int, const DPGroupT*, const DPTermT*,        \
                                      const uint32_t*, const void*, const void*, void*, int, const double*,       \
                                      double*, int64_t, int64_t, bool, bool, hipStream_t);

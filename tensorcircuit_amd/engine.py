@@ -202,6 +202,94 @@ class Program:
         return nb.value
 
 
+def forward_multi(items: Sequence[Tuple[Program, Optional[torch.Tensor], torch.Tensor]],
+                  dtype: str = "complex64", init_zero: bool = True) -> None:
+    """Run many programs (e.g. circuits of different qubit counts) on their
+    states [batch, 2^n] in pass-synchronous launches (tcm_forward_multi)."""
+    dev = device()
+    code = _CODE[dtype]
+    descs = (_lib.TcmMultiDesc * max(len(items), 1))()
+    keep = []
+    for i, (prog, params, state) in enumerate(items):
+        b = state.shape[0]
+        p = None
+        if prog.nparams:
+            p = params.to(device=dev, dtype=_RDT[dtype]).reshape(b, prog.nparams).contiguous()
+            keep.append(p)
+        descs[i].prog = prog.handle.value
+        descs[i].params = None if p is None else p.data_ptr()
+        descs[i].param_stride = prog.nparams
+        descs[i].state = state.data_ptr()
+        descs[i].batch = b
+    nb = _i64()
+    check(lib.tcm_multi_workspace_bytes(descs, len(items), code, C.byref(nb)), "workspace")
+    ws = _workspace(nb.value, dev)
+    check(lib.tcm_forward_multi(descs, len(items), code, 1 if init_zero else 0, ws.data_ptr(), ws.numel(),
+                                _stream()), "tcm_forward_multi")
+
+
+_TUNE: Dict[str, int] = {}
+_TUNE_FILE = None
+
+
+def _tune_path() -> str:
+    import os
+
+    d = os.environ.get("TCM_CACHE_DIR") or os.path.join(os.path.expanduser("~"), ".cache", "tensorcircuit_amd")
+    return os.path.join(d, "autotune.json")
+
+
+def autotune_tile(n: int, specs: Sequence[GateSpec], nparams: int, batch: int, dtype: str = "complex64",
+                  candidates: Sequence[int] = (10, 11, 12, 13), reps: int = 2, **opts: Any) -> int:
+    """Pick the forward tile size (log2 amplitudes per LDS tile) by timing the
+    real program on the real batch; results are cached on disk keyed by
+    (n, dtype, batch, structure).  Staging cost, reported apart from running
+    time like the reference's jit staging (utils.py:205-232)."""
+    import hashlib
+    import os
+
+    global _TUNE_FILE
+    h = hashlib.sha1(repr([s.key() for s in specs]).encode()).hexdigest()[:16]
+    key = "%d/%s/%d/%s" % (n, dtype, batch, h)
+    if _TUNE_FILE is None:
+        _TUNE_FILE = _tune_path()
+        try:
+            with open(_TUNE_FILE) as f:
+                _TUNE.update(json.load(f))
+        except Exception:
+            pass
+    if key in _TUNE:
+        return _TUNE[key]
+    dev = device()
+    rdt = _RDT[dtype]
+    p = torch.zeros(batch, max(nparams, 1), dtype=rdt, device=dev)[:, :nparams] if nparams else None
+    st = torch.empty(batch, 1 << n, dtype=_CDT[dtype], device=dev)
+    best, best_t = 0, float("inf")
+    for k in candidates:
+        if k > n:
+            continue
+        prog = Program(n, specs, nparams, fuse=1, tile_bits=k, **opts)
+        prog.forward(p, batch, dtype, out=st)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            prog.forward(p, batch, dtype, out=st)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1)
+        if t < best_t:
+            best, best_t = k, t
+    _TUNE[key] = best
+    try:
+        os.makedirs(os.path.dirname(_TUNE_FILE), exist_ok=True)
+        with open(_TUNE_FILE, "w") as f:
+            json.dump(_TUNE, f)
+    except Exception:
+        pass
+    return best
+
+
 _PROG_CACHE: "collections.OrderedDict[Tuple, Program]" = collections.OrderedDict()
 _PROG_LOCK = threading.Lock()
 _PROG_CACHE_MAX = 128

@@ -546,3 +546,34 @@ def test_brickwork_fused_mfma_vs_c128(gpu):
                 ref = sv.apply_gate(ref, g.mat, g.qubits, n)
             np.testing.assert_allclose(np_of(s64).reshape(-1), ref, atol=1e-12)
             np.testing.assert_allclose(np_of(s32).reshape(-1), ref, atol=2e-6)
+
+
+# ------------------------------------------------------ batched uneven (C3) ---
+
+def test_forward_multi_uneven(gpu):
+    """Circuits of 10..17 qubits (and a batch of one program) in pass-synchronous
+    multi-program launches equal their single-program runs and the oracle."""
+    from tensorcircuit_amd import Circuit, engine, workloads
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    items, singles, refs = [], [], []
+    for i, n in enumerate(range(10, 18)):
+        L = 3
+        c = Circuit(n)
+        hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
+        specs, thetas = c._lower()
+        prog = engine.get_program(n, specs, len(thetas), fuse=1)
+        b = 1 + i % 3
+        p = torch.as_tensor(workloads.hea_params(2000 + n, L, n, batch=b).reshape(b, -1)).cuda()
+        st = torch.empty(b, 1 << n, dtype=torch.complex64, device="cuda")
+        items.append((prog, p, st))
+        s1, _ = prog.forward(p, b, "complex64")
+        singles.append(s1)
+        if n <= 14:
+            refs.append((n, sv.run_ops(n, sv.hea_ops(n, L), p[0].double().cpu().numpy().reshape(-1))))
+    engine.forward_multi(items)
+    for (prog, p, st), s1 in zip(items, singles):
+        np.testing.assert_allclose(np_of(st), np_of(s1), atol=1e-6)
+    for n, ref in refs:
+        st = [it[2] for it in items if it[0].n == n][0]
+        np.testing.assert_allclose(np_of(st)[0], ref, atol=2e-6)
