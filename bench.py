@@ -23,6 +23,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_fused_adj launch measured with rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE on the default workload (24q, 20 layers, batch 32, 11-bit adjoint
+# tiles), FETCH scaled x2.0 and WRITE x1.0 by a 4 GiB copy of the same run;
+# the commands and per-counter figures are in profiles/k_fused_adj_hea24_b32.md.
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5): 17046440436}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -50,8 +56,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the batch shards across GPUs with no exchange "
-              "(see DESIGN.md), so --gpus %d is not run" % args.gpus, file=sys.stderr)
+        print("bench.py: one GPU only; the batch of parameter sets would shard across GPUs with no "
+              "exchange, so --gpus %d is not run" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -184,9 +190,12 @@ def main():
         fwd_bytes = 2 * bpa * N * B   # read + write psi per pass
         adj_gbs = adj_bytes / (adj_launch_ms * 1e-3) / 1e9
         fwd_gbs = fwd_bytes / (fwd_launch_ms * 1e-3) / 1e9
+        traffic = None
+        if args.adj_tile in (0, 11):
+            traffic = MEASURED_ADJ_TRAFFIC.get((n, L, B, args.ckpt))
         out["roofline"] = {"bound": "hbm", "kernel": "k_fused_adj", "achieved": round(adj_gbs, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
-                           "traffic": None, "bytes_per_launch": adj_bytes,
+                           "traffic": traffic, "bytes_per_launch": adj_bytes,
                            "avg_launch_ms": round(adj_launch_ms, 4), "launches_per_step": n_adj}
         out["roofline_fwd"] = {"kernel": "k_fused_fwd", "achieved": round(fwd_gbs, 1), "unit": "GB/s",
                                "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": fwd_bytes,

@@ -30,4 +30,7 @@ Pinning (what checks each function; details in ``tests/test_oracle.py``)
   (tests/test_interfaces.py:30-97).
 * ``tfim`` energies: exact diagonalisation lower bound at n <= 10, all-zero
   angles give E = -(n-1) exactly.
+* ``haar_su2`` / ``brickwork_ops``: unitarity and det = 1 of every draw.
+* ``plain_torch``: energy and autograd gradient equal ``statevector``'s
+  adjoint at complex128 (5 qubits, 2 layers) to 1e-12.
 """

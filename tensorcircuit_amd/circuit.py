@@ -362,6 +362,30 @@ class Circuit:
     def to_qir(self) -> List[Dict[str, Any]]:
         return self._qir
 
+    @classmethod
+    def from_qir(cls, qir: List[Dict[str, Any]], circuit_params: Optional[Dict[str, Any]] = None) -> "Circuit":
+        """Rebuild a circuit from ``to_qir()`` output (abstractcircuit.py:367-426);
+        without ``nqubits`` in ``circuit_params`` the width is 1 + the largest
+        index used."""
+        params = dict(circuit_params or {})
+        if "nqubits" not in params:
+            params["nqubits"] = 1 + max([max(d["index"]) for d in qir if len(d.get("index", ()))] or [0])
+        c = cls(**params)
+        for d in qir:
+            gatef = d.get("gatef")
+            index = tuple(d.get("index", ()))
+            if gatef is None and d.get("name") == "barrier":
+                c.barrier_instruction()
+            elif isinstance(gatef, G._GateFactory):
+                c._apply_named(gatef.n, index, dict(d.get("parameters", {})), d.get("name", gatef.n),
+                               d.get("split"))
+            elif gatef is None:
+                c.apply_general_gate(d["gate"], *index, name=d.get("name"), split=d.get("split"))
+            else:
+                c.apply_general_variable_gate_delayed(gatef, d.get("name"), mpo=d.get("mpo", False))(
+                    c, *index, **dict(d.get("parameters", {})))
+        return c
+
     def gate_count(self, gate_list: Optional[Union[str, Sequence[str]]] = None) -> int:
         """abstractcircuit.py gate_count."""
         if gate_list is None:

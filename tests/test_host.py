@@ -193,6 +193,29 @@ def test_circuit_records_reference_qir():
         c.expectation_ps(x=[0], z=[0])
 
 
+def test_qir_round_trip():
+    """from_qir(to_qir()) records the same lowered gates (abstractcircuit.py:367-426)."""
+    from tensorcircuit_amd import Circuit, gates
+
+    c = Circuit(4)
+    c.h(0)
+    c.rx(1, theta=0.7)
+    c.exp1(0, 1, unitary=gates._zz_matrix, theta=-0.2)
+    c.u(2, theta=0.1, phi=0.2, lbd=0.3)
+    c.unitary(3, 1, unitary=np.kron(gates._x_matrix, gates._h_matrix))
+    c.apply_general_gate(gates.cz(), 2, 3)
+    c.barrier_instruction()
+    c.toffoli(0, 1, 2)
+    d = Circuit.from_qir(c.to_qir())
+    assert d._nqubits == 4 and len(d._ops) == len(c._ops)
+    for a, b in zip(c._ops, d._ops):
+        assert a.kind == b.kind and a.qubits == b.qubits and a.scale == b.scale
+        assert float(a.theta or 0) == float(b.theta or 0)
+        if a.mat is not None:
+            np.testing.assert_allclose(np.asarray(a.mat), np.asarray(b.mat), atol=0)
+    assert [q["name"] for q in d.to_qir()] == [q["name"] for q in c.to_qir()]
+
+
 def test_fusion_matches_unfused_oracle():
     """fuse() multiplies gates into <=5-qubit dense blocks; replaying the
     blocks through the oracle reproduces the unfused circuit."""

@@ -361,3 +361,43 @@ def test_interface_gradient_kat_2():
     terms = [(1.0, [0, 1, 0, 0]), (-1.0, [0, 0, 1, 0])]
     _, g = sv.adjoint_value_and_grad(n, ops, p, terms)
     np.testing.assert_allclose(g[0], -0.41609, atol=1e-5)
+
+
+def test_plain_torch_baseline_matches_oracle():
+    """The timed CPU baseline (torch tensordot + autograd) computes the same
+    HEA/TFIM energy and gradient as the complex128 oracle."""
+    torch = pytest.importorskip("torch")
+    from oracle import plain_torch
+
+    n, L = 5, 2
+    rng = np.random.default_rng(1001)
+    p = rng.uniform(0, 2 * np.pi, size=(2 * L, n))
+    e_ref, g_ref = sv.adjoint_value_and_grad(n, sv.hea_ops(n, L), p.reshape(-1), sv.tfim_terms(n))
+    pt = torch.tensor(p, dtype=torch.float64)
+    g, e = torch.func.grad_and_value(
+        lambda q: plain_torch.hea_tfim_energy(q, n, L, dtype=torch.complex128))(pt)
+    np.testing.assert_allclose(float(e), e_ref, atol=1e-12)
+    np.testing.assert_allclose(g.numpy().reshape(-1), g_ref, atol=1e-12)
+
+
+def test_param_gate_closed_forms():
+    """Each parametrised matrix equals an independent construction: scipy's
+    expm of its generator, or a product of rotations up to the stated
+    global phase (a sign or index slip in any factory fails here)."""
+    from scipy.linalg import expm
+
+    X, Y, Z = sv._X, sv._Y, sv._Z
+    th, phi, lbd, alpha = 0.731, 0.2, -0.4, 1.1
+    for name, gen in (("rx", X), ("ry", Y), ("rz", Z), ("rxx", np.kron(X, X)), ("ryy", np.kron(Y, Y)),
+                      ("rzz", np.kron(Z, Z))):
+        np.testing.assert_allclose(sv.gate_matrix(name, theta=th), expm(-0.5j * th * gen), atol=1e-12)
+    rz = lambda a: expm(-0.5j * a * Z)  # noqa: E731
+    ry = lambda a: expm(-0.5j * a * Y)  # noqa: E731
+    np.testing.assert_allclose(sv.gate_matrix("u", theta=th, phi=phi, lbd=lbd),
+                               np.exp(0.5j * (phi + lbd)) * rz(phi) @ ry(th) @ rz(lbd), atol=1e-12)
+    np.testing.assert_allclose(sv.phase(th), np.exp(0.5j * th) * rz(th), atol=1e-12)
+    nvec = (np.sin(alpha) * np.cos(phi), np.sin(alpha) * np.sin(phi), np.cos(alpha))
+    gen = nvec[0] * X + nvec[1] * Y + nvec[2] * Z
+    np.testing.assert_allclose(sv.r_gate(th, alpha, phi), expm(-1j * th * gen), atol=1e-12)
+    np.testing.assert_allclose(sv.exp1(np.kron(Z, Z), th), expm(-1j * th * np.kron(Z, Z)), atol=1e-12)
+    np.testing.assert_allclose(sv.expm_herm(gen, th), expm(-1j * th * gen), atol=1e-12)
