@@ -38,6 +38,48 @@ __device__ __forceinline__ uint32_t swz(uint32_t t) {
   return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu);
 }
 
+// Offset (in amplitudes, from the block base) of tile element x: the tile's
+// L low bits are contiguous in the state and the run-offset table `ro` maps
+// the tile bits above them to their state bit positions.  The map is linear
+// over disjoint bit sets, so element it * nthr + tid sits at
+// tile_dep(it * nthr) (block-uniform, scalar loads) + tile_dep(tid).
+__device__ __forceinline__ uint32_t tile_dep(const uint32_t* ro, int L, uint32_t x) {
+  return ro[x >> L] + (x & ((1u << L) - 1));
+}
+
+// a pointer known to be the same in every lane, moved to scalar registers
+template <typename P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<P*>(((uint64_t)hi << 32) | lo);
+}
+
+// A block-uniform zero the optimiser cannot see through: addresses rebuilt
+// from it are recomputed (cheap scalar work) instead of held live in
+// registers across the group loop.
+__device__ __forceinline__ uint32_t opaque_zero() {
+  uint32_t z = 0;
+  asm volatile("" : "+v"(z));
+  return __builtin_amdgcn_readfirstlane(z);
+}
+__device__ __forceinline__ void rematerialize(uint32_t& dlo, uint32_t& slo, const uint32_t*& ro) {
+  asm volatile("" : "+v"(dlo), "+v"(slo));
+  ro += opaque_zero();
+}
+
+// tile position of a group's thread bits: the thread index bits moved as
+// runs of consecutive bits to their tile positions
+__device__ __forceinline__ uint32_t group_base(const DGroup* G, uint32_t tid) {
+  uint32_t tb = 0;
+  for (int r = 0; r < G->nrun; ++r) {
+    const uint32_t w = (uint32_t)G->run[r];
+    tb |= ((tid >> (w & 0xFFu)) & ((1u << ((w >> 8) & 0xFFu)) - 1u)) << (w >> 16);
+  }
+  return tb;
+}
+
 // insert a zero bit at position b
 __device__ __forceinline__ uint64_t ins0(uint64_t x, int b) {
   uint64_t lo = x & ((1ull << b) - 1);
@@ -797,37 +839,40 @@ __device__ __forceinline__ void fused_fwd_tile(const DPass* P, const DGroup* __r
   const int tid = threadIdx.x;
   const bool act = tid < nthr;
   const uint32_t* ro = runoff + P->runoff0;
-  const uint32_t lmask = (1u << L) - 1;
+  uint32_t dlo = act ? tile_dep(ro, L, tid) : 0u;
+  uint32_t slo = swz(tid);
   if (act) {
+    // all loads in flight before the first LDS write
+    V2<T> buf[1 << R];
 #pragma unroll
-    for (int it = 0; it < (1 << R); ++it) {
-      const uint32_t a = it * nthr + tid;
-      tile[swz(a)] = s[ro[a >> L] + (a & lmask)];
-    }
+    for (int it = 0; it < (1 << R); ++it) buf[it] = s[tile_dep(ro, L, it * nthr) + dlo];
+#pragma unroll
+    for (int it = 0; it < (1 << R); ++it) tile[swz(it * nthr) ^ slo] = buf[it];
   }
   __syncthreads();
   for (int g = 0; g < P->ngroups; ++g) {
     const DGroup* G = groups + P->group0 + g;
     if (act) {
-      uint32_t tb = 0;
-      for (int i = 0; i < k - R; ++i)
-        if ((tid >> i) & 1) tb |= 1u << G->cb[i];
-      const uint32_t a0 = swz(tb);
+      const uint32_t a0 = swz(group_base(G, tid));
       V2<T> v[1 << R];
 #pragma unroll
       for (int j = 0; j < (1 << R); ++j) v[j] = tile[a0 ^ (uint32_t)G->S[j]];
       for (int o = 0; o < G->nops; ++o) apply_op<T, R, false, FULL>(v, ops[G->op0 + o], mt, ct);
+      const DGroup* Gs = G + opaque_zero();
+      uint32_t as = a0;
+      asm volatile("" : "+v"(as));
 #pragma unroll
-      for (int j = 0; j < (1 << R); ++j) tile[a0 ^ (uint32_t)G->S[j]] = v[j];
+      for (int j = 0; j < (1 << R); ++j) tile[as ^ (uint32_t)Gs->S[j]] = v[j];
     }
     __syncthreads();
   }
+  rematerialize(dlo, slo, ro);
   if (act) {
 #pragma unroll
     for (int it = 0; it < (1 << R); ++it) {
-      const uint32_t a = it * nthr + tid;
-      const V2<T> x = tile[swz(a)];
-      const uint32_t off = ro[a >> L] + (a & lmask);
+      const uint32_t ah = it * nthr;
+      const V2<T> x = tile[swz(ah) ^ slo];
+      const uint32_t off = tile_dep(ro, L, ah) + dlo;
       s[off] = x;
       if (c) c[off] = x;
     }
@@ -859,21 +904,27 @@ __global__ void __launch_bounds__(TB) k_fused_fwd_multi(MultiJobs J) {
   extern __shared__ __align__(16) unsigned char smem[];
   int j = 0;
   while (j + 1 < J.njobs && (int64_t)blockIdx.x >= J.job[j + 1].block0) ++j;
+  j = __builtin_amdgcn_readfirstlane(j);  // block-uniform
   const MultiJob& Q = J.job[j];
-  const DPass* P = (const DPass*)Q.dpass + Q.pass;
+  // the job's fields are block-uniform: keep them in scalar registers
+  const DPass* P = uniform_ptr((const DPass*)Q.dpass) + __builtin_amdgcn_readfirstlane(Q.pass);
+  const DGroup* groups = uniform_ptr((const DGroup*)Q.dgroup);
+  const DOp* ops = uniform_ptr((const DOp*)Q.dop);
+  const uint32_t* runoff = uniform_ptr((const uint32_t*)Q.runoff);
+  const T* ct = uniform_ptr((const T*)Q.ctab);
   const int nbl = P->nblk_log;
   const uint64_t lb = (uint64_t)blockIdx.x - Q.block0;
   const uint64_t blk = lb & ((1ull << nbl) - 1);
   const uint64_t b = lb >> nbl;
   const uint64_t base = block_base(P, blk);
-  const T* mt = (const T*)Q.mtab + b * Q.mstride;
-  V2<T>* s = (V2<T>*)Q.state + b * Q.sstride + base;
+  const T* mt = uniform_ptr((const T*)Q.mtab + b * Q.mstride);
+  V2<T>* s = uniform_ptr((V2<T>*)Q.state + b * Q.sstride + base);
   if (P->full)
-    fused_fwd_tile<T, R, true>(P, (const DGroup*)Q.dgroup, (const DOp*)Q.dop, (const uint32_t*)Q.runoff, mt,
-                               (const T*)Q.ctab, s, nullptr, 1 << P->tthreads_log, reinterpret_cast<V2<T>*>(smem));
+    fused_fwd_tile<T, R, true>(P, groups, ops, runoff, mt, ct, s, nullptr, 1 << P->tthreads_log,
+                               reinterpret_cast<V2<T>*>(smem));
   else
-    fused_fwd_tile<T, R, false>(P, (const DGroup*)Q.dgroup, (const DOp*)Q.dop, (const uint32_t*)Q.runoff, mt,
-                                (const T*)Q.ctab, s, nullptr, 1 << P->tthreads_log, reinterpret_cast<V2<T>*>(smem));
+    fused_fwd_tile<T, R, false>(P, groups, ops, runoff, mt, ct, s, nullptr, 1 << P->tthreads_log,
+                                reinterpret_cast<V2<T>*>(smem));
 }
 
 // Reverse sweep over one fused pass.  psi/lambda tiles are read from
@@ -904,25 +955,28 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
   const uint64_t base = block_base(P, blk);
   const T* mt = mtab + b * mstride;
   const uint32_t* ro = runoff + P->runoff0;
-  const uint32_t lmask = (1u << L) - 1;
+  uint32_t dlo = tile_dep(ro, L, tid);
+  uint32_t slo = swz(tid);
   {
     const V2<T>* sp = psi_in + b * sstride + base;
     const V2<T>* sl = lam_in + b * sstride + base;
+    V2<T> bp[1 << R], bl[1 << R];
 #pragma unroll
     for (int it = 0; it < (1 << R); ++it) {
-      const uint32_t a = it * nthr + tid;
-      const uint32_t off = ro[a >> L] + (a & lmask);
-      tp[swz(a)] = sp[off];
-      tl[swz(a)] = sl[off];
+      const uint32_t off = tile_dep(ro, L, it * nthr) + dlo;
+      bp[it] = sp[off];
+      bl[it] = sl[off];
+    }
+#pragma unroll
+    for (int it = 0; it < (1 << R); ++it) {
+      tp[swz(it * nthr) ^ slo] = bp[it];
+      tl[swz(it * nthr) ^ slo] = bl[it];
     }
   }
   __syncthreads();
   for (int g = P->ngroups - 1; g >= 0; --g) {
     const DGroup* G = groups + P->group0 + g;
-    uint32_t tb = 0;
-    for (int i = 0; i < k - R; ++i)
-      if ((tid >> i) & 1) tb |= 1u << G->cb[i];
-    const uint32_t a0 = swz(tb);
+    const uint32_t a0 = swz(group_base(G, tid));
     V2<T> p[1 << R], l[1 << R];
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
@@ -947,22 +1001,28 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
       apply_op<T, R, true, FULL>(p, op, mt, ct);
       apply_op<T, R, true, FULL>(l, op, mt, ct);
     }
+    const DGroup* Gs = G + opaque_zero();
+    uint32_t as = a0;
+    asm volatile("" : "+v"(as));
 #pragma unroll
     for (int j = 0; j < (1 << R); ++j) {
-      tp[a0 ^ (uint32_t)G->S[j]] = p[j];
-      tl[a0 ^ (uint32_t)G->S[j]] = l[j];
+      tp[as ^ (uint32_t)Gs->S[j]] = p[j];
+      tl[as ^ (uint32_t)Gs->S[j]] = l[j];
     }
     __syncthreads();
   }
+  // recompute the store addresses instead of holding the load addresses in
+  // registers through the group loop
+  rematerialize(dlo, slo, ro);
   {
     V2<T>* sp = psi_out ? psi_out + b * sstride + base : nullptr;
     V2<T>* sl = lam_out + b * sstride + base;
 #pragma unroll
     for (int it = 0; it < (1 << R); ++it) {
-      const uint32_t a = it * nthr + tid;
-      const uint32_t off = ro[a >> L] + (a & lmask);
-      if (sp) sp[off] = tp[swz(a)];
-      sl[off] = tl[swz(a)];
+      const uint32_t ah = it * nthr;
+      const uint32_t off = tile_dep(ro, L, ah) + dlo;
+      if (sp) sp[off] = tp[swz(ah) ^ slo];
+      sl[off] = tl[swz(ah) ^ slo];
     }
   }
   const int ng = P->ngop;
@@ -1210,11 +1270,12 @@ __global__ void __launch_bounds__(256) k_pauli_tile(const DPPass* __restrict__ p
     if ((blk >> i) & 1) base |= 1ull << P->blk_bits[i];
   const V2<T>* s = psi + b * sstride + base;
   const uint32_t* ro = runoff + P->runoff0;
-  const uint32_t lmask = (1u << L) - 1;
+  const uint32_t dlo = tile_dep(ro, L, tid);
+  const uint32_t slo = swz(tid);
   const int per = (1 << k) / nthr;
   for (int it = 0; it < per; ++it) {
-    const uint32_t a = it * nthr + tid;
-    tile[swz(a)] = s[ro[a >> L] + (a & lmask)];
+    const uint32_t ah = it * nthr;
+    tile[swz(ah) ^ slo] = s[tile_dep(ro, L, ah) + dlo];
   }
   __syncthreads();
   const double al = (APPLY && alpha) ? alpha[b] : 1.0;
@@ -1247,7 +1308,7 @@ __global__ void __launch_bounds__(256) k_pauli_tile(const DPPass* __restrict__ p
       ei += (double)p.x * hi - (double)p.y * hr;
     }
     if constexpr (APPLY) {
-      const uint64_t off = b * sstride + base + ro[a >> L] + (a & lmask);
+      const uint64_t off = b * sstride + base + tile_dep(ro, L, it * nthr) + dlo;
       V2<T> o = lam_in ? lam_in[off] : mk<T>(0, 0);
       lam_out[off] = mk<T>((T)(o.x + al * hr), (T)(o.y + al * hi));
     }

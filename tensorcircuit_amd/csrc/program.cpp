@@ -638,6 +638,13 @@ static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant) {
       int nc = 0;
       for (int b = 0; b < dp.k; ++b)
         if (!(sm >> b & 1)) dg.cb[nc++] = b;
+      // thread bits in runs of consecutive tile bits (one bit-field move each)
+      for (int i = 0; i < nc;) {
+        int len = 1;
+        while (i + len < nc && dg.cb[i + len] == dg.cb[i] + len) ++len;
+        dg.run[dg.nrun++] = i | len << 8 | dg.cb[i] << 16;
+        i += len;
+      }
       if (hg.lds_op) {
         dg.lds_op = 1;
         for (size_t i = 0; i < hg.slots.size(); ++i) dg.ukbits[i] = hg.slots[i];

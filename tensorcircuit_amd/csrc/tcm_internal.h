@@ -79,6 +79,8 @@ struct DMember {
 // LDS tile at tile addresses swz(tbase(tid)) ^ S[j].
 struct DGroup {
   int32_t cb[20];    // tile-local bit positions of the thread bits (k - R)
+  int32_t nrun;      // cb as runs of consecutive bits: run[r] = src | len << 8 | dst << 16
+  int32_t run[20];   // (thread bits src .. src+len-1 -> tile bits dst ..)
   int32_t S[32];     // swizzled tile offsets of the 2^R register slots
   int32_t op0, nops;
   int32_t lds_op;    // 1: group holds a single OK_UK op applied on the LDS tile
