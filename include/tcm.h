@@ -91,7 +91,11 @@ typedef struct tcm_opts {
   int32_t reg_slots;   /* register-resident qubits per thread; 0 = auto (4) */
   int32_t mfma;        /* 1: dense 5-qubit gates run on FP32 matrix cores
                           (complex64 only; complex128 keeps the VALU kernel) */
-  int32_t reserved[2];
+  int32_t jit;         /* 1: compile every tile pass into its own kernel
+                          (straight-line code, hiprtc) on first use of a
+                          dtype/mode; identical passes share a kernel and
+                          code objects are cached on disk */
+  int32_t reserved;
 } tcm_opts;
 
 typedef struct tcm_program tcm_program; /* opaque, caller-owned */
@@ -124,6 +128,14 @@ int tcm_program_get_info(const tcm_program* prog, tcm_program_info* info);
  * into buf (NUL-terminated, truncated to cap); returns the full length. */
 int64_t tcm_program_describe(const tcm_program* prog, int32_t dtype,
                              int32_t mode, char* buf, int64_t cap);
+
+/* Pass compiler (opts.jit = 1, no reference counterpart): compile the
+ * schedule for dtype and mode (0 forward, 1 adjoint) now instead of on first
+ * use; needs no device (targets the current device's architecture, else
+ * gfx950).  *seconds (nullable) receives the compile time, 0 when the code
+ * objects came from the on-disk cache. */
+int tcm_program_compile(tcm_program* prog, int32_t dtype, int32_t mode,
+                        double* seconds);
 
 /* Scratch bytes tcm_forward needs for `batch` states (per-item gate
  * matrices evaluated from the parameters). */

@@ -43,7 +43,8 @@ class TcmOpts(C.Structure):
         ("ckpt_every", C.c_int32),
         ("reg_slots", C.c_int32),
         ("mfma", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("jit", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -85,6 +86,7 @@ SIGNATURES = {
     "tcm_program_free": (C.c_int, [_vp]),
     "tcm_program_get_info": (C.c_int, [_vp, C.POINTER(TcmProgramInfo)]),
     "tcm_program_describe": (C.c_int64, [_vp, _i32, _i32, C.c_char_p, _i64]),
+    "tcm_program_compile": (C.c_int, [_vp, _i32, _i32, C.POINTER(C.c_double)]),
     "tcm_forward_workspace_bytes": (C.c_int, [_vp, _i32, _i64, C.POINTER(_i64)]),
     "tcm_forward": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _i32, _vp, _vp, _i64, _vp]),
     "tcm_pauli_build": (C.c_int, [C.POINTER(C.c_int8), _dp, _i32, _i32, C.POINTER(_vp)]),

@@ -105,6 +105,8 @@ void free_tables(DeviceTables* D) {
                   D->gop_list, D->ctab, D->p_ptr, D->p_gops, D->gfac};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (void* m : D->jit_mod)
+    if (m) (void)hipModuleUnload((hipModule_t)m);
   delete D;
 }
 
@@ -115,6 +117,9 @@ int ensure_tables(Program& P, int dtype, int mode, DeviceTables** out) {
   Sched& S = P.sched[dtype][mode];
   if (S.k > 0 && S.dpass.size() != S.passes.size())
     return set_error(TCM_E_ARG, "internal: schedule has %zu passes but %zu descriptors", S.passes.size(), S.dpass.size());
+  if (P.opts.jit && S.k > 0 && S.jit_state == 0)
+    if (int rc = jit_build(P, S, dtype, mode)) return rc;
+  if (S.jit_state < 0) return set_error(TCM_E_UNSUPPORTED, "pass compiler failed for this program");
   if ((int)S.dev.size() <= dev) S.dev.resize(dev + 1, nullptr);
   if (S.dev[dev]) {
     *out = S.dev[dev];
@@ -145,6 +150,7 @@ int ensure_tables(Program& P, int dtype, int mode, DeviceTables** out) {
   rc = rc ? rc : upload(P.p_ptr, &D->p_ptr);
   rc = rc ? rc : upload(P.p_gops, &D->p_gops);
   rc = rc ? rc : upload(P.gfac, &D->gfac);
+  rc = rc ? rc : jit_load(S, *D);
   if (rc) {
     free_tables(D);
     return rc;
@@ -196,8 +202,13 @@ int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const 
           else extra.push_back(c);
         }
     if (tiled) {
-      if (S.R == 4) launch_fused_fwd<T, 4>(S, D, pi, mtab, D.ctab, state, batch, n, ckout, st);
-      else return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
+      if (!S.jit.empty() && S.jit[pi].kernel >= 0) {
+        if (int rc = jit_launch_fwd(S, D, pi, sizeof(T) == 8, mtab, D.ctab, state, batch, n, ckout, st)) return rc;
+      } else if (S.R == 4) {
+        launch_fused_fwd<T, 4>(S, D, pi, mtab, D.ctab, state, batch, n, ckout, st);
+      } else {
+        return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
+      }
     } else {
       const DOp& op = S.k > 0 ? S.dop[S.dpass[pi].plain_op] : S.dop[pi];
       launch_plain_gate<T, false>(op, (const int32_t*)D.ukbits, mtab, S.mtab_size, D.ctab, state, n, batch, st);
@@ -467,10 +478,15 @@ int reverse_sweep(Program& P, int dtype, const AdjWs& w, const void* psi_src, co
       }
       void* pout = c >= 0 ? nullptr : w.psi;
       void* lout = (p == 0 && grad_input) ? grad_input : w.lam;
-      if (A.R == 4)
+      if (!A.jit.empty() && A.jit[p].kernel >= 0) {
+        if (int rc = jit_launch_adj(A, *D, p, sizeof(T) == 8, w.mtab_a, D->ctab, pin, pout, lin, lout, batch, n,
+                                    w.part, pstride, st))
+          return rc;
+      } else if (A.R == 4) {
         launch_fused_adj<T, 4>(A, *D, p, w.mtab_a, D->ctab, pin, pout, lin, lout, batch, n, w.part, pstride, st);
-      else
+      } else {
         return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", A.R);
+      }
       if (dp.ngop > 0)
         launch_reduce_gop(w.part, pstride, 1ll << dp.nblk_log, (const int32_t*)D->gop_list + dp.gop_list0, 0,
                           dp.ngop, w.gsum, P.ngop, batch, st);
@@ -677,6 +693,19 @@ int tcm_program_get_info(const tcm_program* prog, tcm_program_info* info) {
   info->npasses = (int)P.sched[0][0].passes.size();
   info->nckpt = (int)P.ckpt_gate.size();
   info->tile_bits = P.sched[0][0].k;
+  return 0;
+}
+
+int tcm_program_compile(tcm_program* prog, int32_t dtype, int32_t mode, double* seconds) {
+  if (!prog || !valid_dtype(dtype) || (mode != 0 && mode != 1)) return set_error(TCM_E_ARG, "bad compile args");
+  Program& P = prog->p;
+  std::lock_guard<std::mutex> lk(P.mu);
+  Sched& S = P.sched[dtype][mode];
+  if (!P.opts.jit) return set_error(TCM_E_UNSUPPORTED, "program built without opts.jit");
+  if (S.k > 0 && S.jit_state == 0)
+    if (int rc = jit_build(P, S, dtype, mode)) return rc;
+  if (S.jit_state < 0) return set_error(TCM_E_UNSUPPORTED, "pass compiler failed for this program");
+  if (seconds) *seconds = S.jit_seconds;
   return 0;
 }
 

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "tcm_internal.h"
 
 namespace tcm {
@@ -85,7 +87,18 @@ struct DeviceTables {
   void* p_ptr = nullptr;
   void* p_gops = nullptr;
   void* gfac = nullptr;
+  // pass compiler: loaded modules and kernel handles (hipModule_t / hipFunction_t)
+  std::vector<void*> jit_mod;
+  std::vector<void*> jit_fn;
 };
+
+// launch compiled pass kernels (jit.cpp)
+int jit_load(const Sched& S, DeviceTables& D);
+int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const void* mtab, const void* ctab,
+                   void* state, int64_t batch, int n, void* ckout, hipStream_t st);
+int jit_launch_adj(const Sched& S, const DeviceTables& D, int pi, int dt, const void* mtab, const void* ctab,
+                   const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n,
+                   double* part, int64_t pstride, hipStream_t st);
 
 template <typename T>
 void launch_prep(const Sched& S, const DeviceTables& D, const void* ctab, const void* params, int64_t pstride,

@@ -1,0 +1,586 @@
+// Pass compiler, part 1: HIP source text for the tile passes of a schedule.
+//
+// The generic fused kernels (kernels.hip) read each pass's gate list from
+// device tables and branch on op kind and register slot per op.  Here the
+// same pass is written out as straight-line code: tile/bit mappings,
+// register slots and table offsets are compile-time constants, permutation
+// gates (X, CNOT, SWAP) become register renames, and there is no per-op
+// dispatch.  The arithmetic of every op and gradient is the generic
+// kernel's, term for term.
+//
+// Passes with the same body (layered circuits repeat them with shifted
+// per-item matrix offsets) share one kernel: offsets into the per-item
+// table are emitted relative to the pass's smallest offset, which the
+// launcher passes as `mbase`.
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "tcm_internal.h"
+
+namespace tcm {
+
+namespace {
+
+std::string fmt(const char* f, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+uint32_t swz_h(uint32_t t) { return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu); }
+
+// OR of bit fields: source bits src[i] -> destination bits dst[i], grouped
+// into runs of consecutive positions.
+std::string deposit_expr(const char* x, const std::vector<int>& src, const std::vector<int>& dst, const char* ty) {
+  std::string e;
+  size_t i = 0;
+  while (i < src.size()) {
+    size_t len = 1;
+    while (i + len < src.size() && src[i + len] == src[i] + (int)len && dst[i + len] == dst[i] + (int)len) ++len;
+    const unsigned long long mask = (len >= 64) ? ~0ull : ((1ull << len) - 1);
+    const bool wide = std::string(ty).find("long") != std::string::npos;
+    std::string term = fmt("(((%s)(%s) >> %d) & 0x%llx%s)", ty, x, src[i], mask, wide ? "ull" : "u");
+    if (dst[i] > 0) term = fmt("(%s << %d)", term.c_str(), dst[i]);
+    e += (e.empty() ? "" : " | ") + term;
+    i += len;
+  }
+  return e.empty() ? std::string("0") : e;
+}
+
+const char* kPrelude = R"(
+typedef REAL T;
+struct __attribute__((aligned(2 * sizeof(REAL)))) cT { T x, y; };
+__device__ __forceinline__ unsigned swz(unsigned t) {
+  return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu);
+}
+// LDS element at a byte offset (addresses are kept in bytes so the XOR
+// swizzle needs no per-access shift)
+#define LDSB(arr, off) (*reinterpret_cast<cT*>(reinterpret_cast<char*>(arr) + (off)))
+__device__ __forceinline__ T imcj(cT a, cT b) { return a.x * b.y - a.y * b.x; }
+__device__ __forceinline__ T recj(cT a, cT b) { return a.x * b.x + a.y * b.y; }
+union tcm_d2i { double d; int i[2]; };
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr(double x) {
+  tcm_d2i u, r;
+  u.d = x;
+  r.i[0] = __builtin_amdgcn_update_dpp(0, u.i[0], CTRL, 0xF, 0xF, false);
+  r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], CTRL, 0xF, 0xF, false);
+  return r.d;
+}
+__device__ __forceinline__ double read_lane(double x, int lane) {
+  tcm_d2i u, r;
+  u.d = x;
+  r.i[0] = __builtin_amdgcn_readlane(u.i[0], lane);
+  r.i[1] = __builtin_amdgcn_readlane(u.i[1], lane);
+  return r.d;
+}
+// fp64 sum over a 64-lane wave (uniform result)
+__device__ __forceinline__ double wsum(double x) {
+  x += dpp_shr<0x111>(x);
+  x += dpp_shr<0x112>(x);
+  x += dpp_shr<0x114>(x);
+  x += dpp_shr<0x118>(x);
+  return (read_lane(x, 15) + read_lane(x, 31)) + (read_lane(x, 47) + read_lane(x, 63));
+}
+)";
+
+struct Gen {
+  const Sched& S;
+  const DPass& dp;
+  const HPass& hp;
+  bool dag;       // adjoint: apply U^dagger, psi and lambda arrays
+  int R, nreg;
+  int mbase;
+  std::string out;
+  std::vector<int> where;  // logical register index -> variable index
+
+  Gen(const Sched& s, const DPass& d, const HPass& h, bool adj) : S(s), dp(d), hp(h), dag(adj), R(s.R) {
+    nreg = 1 << R;
+    mbase = -1;
+  }
+
+  void line(const std::string& s) { out += s; out += '\n'; }
+
+  // matrix element source for an op's table offset `moff` (+i)
+  std::string mel(int moff, int i) const {
+    if (moff >= 0) return fmt("mt[%d]", moff - mbase + i);
+    return fmt("ct[%d]", -moff - 1 + i);
+  }
+  std::string var(const char* a, int j) const { return fmt("%s%d", a, where[j]); }
+
+  // ---- ops on one register array named `a` (DAG per `dag`) ----
+  void op_u1(const char* a, int S0, const std::string& pre) {
+    const int m = 1 << S0;
+    for (int j = 0; j < nreg; ++j) {
+      if (j & m) continue;
+      const std::string x = var(a, j), y = var(a, j | m);
+      line(fmt("{ const cT x = %s, y = %s;", x.c_str(), y.c_str()));
+      line(fmt("  %s = cT{%sar * x.x - %sai * x.y + %sbr * y.x - %sbi * y.y, %sar * x.y + %sai * x.x + %sbr * y.y + %sbi * y.x};",
+               x.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(),
+               pre.c_str()));
+      line(fmt("  %s = cT{%scr * x.x - %sci * x.y + %sdr * y.x - %sdi * y.y, %scr * x.y + %sci * x.x + %sdr * y.y + %sdi * y.x}; }",
+               y.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(),
+               pre.c_str()));
+    }
+  }
+  void op_r1(const char* a, int S0, const std::string& pre) {
+    const int m = 1 << S0;
+    for (int j = 0; j < nreg; ++j) {
+      if (j & m) continue;
+      const std::string x = var(a, j), y = var(a, j | m);
+      const char* P = pre.c_str();
+      line(fmt("{ const cT x = %s, y = %s;", x.c_str(), y.c_str()));
+      line(fmt("  %s = cT{%sa * x.x + %sb * y.x, %sa * x.y + %sb * y.y};", x.c_str(), P, P, P, P));
+      line(fmt("  %s = cT{%sc * x.x + %sd * y.x, %sc * x.y + %sd * y.y}; }", y.c_str(), P, P, P, P));
+    }
+  }
+  void op_d1(const char* a, int S0, const std::string& pre) {
+    const int m = 1 << S0;
+    const char* P = pre.c_str();
+    for (int j = 0; j < nreg; ++j) {
+      const std::string x = var(a, j);
+      const char* d = (j & m) ? "1" : "0";
+      line(fmt("{ const cT x = %s; %s = cT{%sd%sr * x.x - %sd%si * x.y, %sd%sr * x.y + %sd%si * x.x}; }", x.c_str(),
+               x.c_str(), P, d, P, d, P, d, P, d));
+    }
+  }
+  void op_d2(const char* a, int S0, int S1, const std::string& pre) {
+    const char* P = pre.c_str();
+    for (int j = 0; j < nreg; ++j) {
+      const int r = (((j >> S0) & 1) << 1) | ((j >> S1) & 1);
+      const std::string x = var(a, j);
+      line(fmt("{ const cT x = %s; %s = cT{%sd%dr * x.x - %sd%di * x.y, %sd%dr * x.y + %sd%di * x.x}; }", x.c_str(),
+               x.c_str(), P, r, P, r, P, r, P, r));
+    }
+  }
+  void op_dg(const char* a, const std::string& pre) {
+    const char* P = pre.c_str();
+    for (int j = 0; j < nreg; ++j) {
+      const std::string x = var(a, j);
+      line(fmt("{ const cT x = %s; %s = cT{%sg%dr * x.x - %sg%di * x.y, %sg%dr * x.y + %sg%di * x.x}; }", x.c_str(),
+               x.c_str(), P, j, P, j, P, j, P, j));
+    }
+  }
+  void op_u2(const char* a, int S0, int S1, const std::string& pre) {
+    const int m0 = 1 << S0, m1 = 1 << S1;
+    const char* P = pre.c_str();
+    for (int j = 0; j < nreg; ++j) {
+      if (j & (m0 | m1)) continue;
+      const int idx[4] = {j, j | m1, j | m0, j | m0 | m1};
+      line(fmt("{ const cT x0 = %s, x1 = %s, x2 = %s, x3 = %s;", var(a, idx[0]).c_str(), var(a, idx[1]).c_str(),
+               var(a, idx[2]).c_str(), var(a, idx[3]).c_str()));
+      for (int r = 0; r < 4; ++r) {
+        std::string re, im;
+        for (int c = 0; c < 4; ++c) {
+          re += fmt("%s%su%d%dr * x%d.x - %su%d%di * x%d.y", c ? " + " : "", P, r, c, c, P, r, c, c);
+          im += fmt("%s%su%d%dr * x%d.y + %su%d%di * x%d.x", c ? " + " : "", P, r, c, c, P, r, c, c);
+        }
+        line(fmt("  %s = cT{%s, %s};", var(a, idx[r]).c_str(), re.c_str(), im.c_str()));
+      }
+      line("}");
+    }
+  }
+  void op_cz(const char* a, int S0, int S1) {
+    for (int j = 0; j < nreg; ++j)
+      if ((j >> S0 & 1) && (j >> S1 & 1)) {
+        const std::string x = var(a, j);
+        line(fmt("%s = cT{-%s.x, -%s.y};", x.c_str(), x.c_str(), x.c_str()));
+      }
+  }
+
+  // coefficient declarations for an op, prefixed `pre` (DAG-adjusted)
+  void coeffs(const DOp& op, const std::string& pre) {
+    const char* P = pre.c_str();
+    switch (op.kind) {
+      case OK_U1:
+        if (!dag)
+          line(fmt("const T %sar = %s, %sai = %s, %sbr = %s, %sbi = %s, %scr = %s, %sci = %s, %sdr = %s, %sdi = %s;", P,
+                   mel(op.moff, 0).c_str(), P, mel(op.moff, 1).c_str(), P, mel(op.moff, 2).c_str(), P,
+                   mel(op.moff, 3).c_str(), P, mel(op.moff, 4).c_str(), P, mel(op.moff, 5).c_str(), P,
+                   mel(op.moff, 6).c_str(), P, mel(op.moff, 7).c_str()));
+        else  // U^dagger: a* c* / b* d*
+          line(fmt("const T %sar = %s, %sai = -%s, %sbr = %s, %sbi = -%s, %scr = %s, %sci = -%s, %sdr = %s, %sdi = -%s;",
+                   P, mel(op.moff, 0).c_str(), P, mel(op.moff, 1).c_str(), P, mel(op.moff, 4).c_str(), P,
+                   mel(op.moff, 5).c_str(), P, mel(op.moff, 2).c_str(), P, mel(op.moff, 3).c_str(), P,
+                   mel(op.moff, 6).c_str(), P, mel(op.moff, 7).c_str()));
+        break;
+      case OK_R1:
+        line(fmt("const T %sa = %s, %sb = %s, %sc = %s, %sd = %s;", P, mel(op.moff, 0).c_str(), P,
+                 mel(op.moff, dag ? 2 : 1).c_str(), P, mel(op.moff, dag ? 1 : 2).c_str(), P, mel(op.moff, 3).c_str()));
+        break;
+      case OK_D1:
+        line(fmt("const T %sd0r = %s, %sd0i = %s%s, %sd1r = %s, %sd1i = %s%s;", P, mel(op.moff, 0).c_str(), P,
+                 dag ? "-" : "", mel(op.moff, 1).c_str(), P, mel(op.moff, 2).c_str(), P, dag ? "-" : "",
+                 mel(op.moff, 3).c_str()));
+        break;
+      case OK_D2:
+        for (int r = 0; r < 4; ++r)
+          line(fmt("const T %sd%dr = %s, %sd%di = %s%s;", P, r, mel(op.moff, 2 * r).c_str(), P, r, dag ? "-" : "",
+                   mel(op.moff, 2 * r + 1).c_str()));
+        break;
+      case OK_DG:
+        for (int j = 0; j < nreg; ++j)
+          line(fmt("const T %sg%dr = %s, %sg%di = %s%s;", P, j, mel(op.moff, 2 * j).c_str(), P, j, dag ? "-" : "",
+                   mel(op.moff, 2 * j + 1).c_str()));
+        break;
+      case OK_U2:
+        for (int r = 0; r < 4; ++r)
+          for (int c = 0; c < 4; ++c) {
+            const int e = dag ? (c * 4 + r) : (r * 4 + c);
+            line(fmt("const T %su%d%dr = %s, %su%d%di = %s%s;", P, r, c, mel(op.moff, 2 * e).c_str(), P, r, c,
+                     dag ? "-" : "", mel(op.moff, 2 * e + 1).c_str()));
+          }
+        break;
+      default:
+        break;
+    }
+  }
+
+  // apply op to the arrays in `arrs` (one or two names)
+  void apply(const DOp& op, const std::vector<const char*>& arrs, int id) {
+    const std::string pre = fmt("c%d_", id);
+    coeffs(op, pre);
+    switch (op.kind) {
+      case OK_U1: for (auto a : arrs) op_u1(a, op.s0, pre); break;
+      case OK_R1: for (auto a : arrs) op_r1(a, op.s0, pre); break;
+      case OK_D1: for (auto a : arrs) op_d1(a, op.s0, pre); break;
+      case OK_D2: for (auto a : arrs) op_d2(a, op.s0, op.s1, pre); break;
+      case OK_DG: for (auto a : arrs) op_dg(a, pre); break;
+      case OK_U2: for (auto a : arrs) op_u2(a, op.s0, op.s1, pre); break;
+      case OK_CZ: for (auto a : arrs) op_cz(a, op.s0, op.s1); break;
+      case OK_X1: {
+        const int m = 1 << op.s0;
+        for (int j = 0; j < nreg; ++j)
+          if (!(j & m)) std::swap(where[j], where[j | m]);
+      } break;
+      case OK_CX: {
+        const int c = 1 << op.s0, t = 1 << op.s1;
+        for (int j = 0; j < nreg; ++j)
+          if ((j & c) && !(j & t)) std::swap(where[j], where[j | t]);
+      } break;
+      case OK_SWAP: {
+        const int a0 = 1 << op.s0, a1 = 1 << op.s1;
+        for (int j = 0; j < nreg; ++j)
+          if ((j & a0) && !(j & a1)) std::swap(where[j], where[(j ^ a0) | a1]);
+      } break;
+      default:
+        break;
+    }
+  }
+
+  std::string pv(int j) const { return var("p", j); }
+  std::string lv(int j) const { return var("l", j); }
+
+  // gradient Im<l|G|p> of a generator on the registers; returns expression code into `ga`
+  bool grad(int gen, int s0, int s1, int goff) {
+    line("T ga = 0;");
+    switch (gen) {
+      case GN_X: {
+        const int m = 1 << s0;
+        for (int j = 0; j < nreg; ++j)
+          if (!(j & m))
+            line(fmt("ga += imcj(%s, %s) + imcj(%s, %s);", lv(j).c_str(), pv(j | m).c_str(), lv(j | m).c_str(),
+                     pv(j).c_str()));
+      } break;
+      case GN_Y: {
+        const int m = 1 << s0;
+        for (int j = 0; j < nreg; ++j)
+          if (!(j & m))
+            line(fmt("ga += recj(%s, %s) - recj(%s, %s);", lv(j | m).c_str(), pv(j).c_str(), lv(j).c_str(),
+                     pv(j | m).c_str()));
+      } break;
+      case GN_Z:
+        for (int j = 0; j < nreg; ++j)
+          line(fmt("ga %s= imcj(%s, %s);", (j >> s0 & 1) ? "-" : "+", lv(j).c_str(), pv(j).c_str()));
+        break;
+      case GN_P1:
+        for (int j = 0; j < nreg; ++j)
+          if (j >> s0 & 1) line(fmt("ga += imcj(%s, %s);", lv(j).c_str(), pv(j).c_str()));
+        break;
+      case GN_XX: {
+        const int m = (1 << s0) | (1 << s1);
+        for (int j = 0; j < nreg; ++j) line(fmt("ga += imcj(%s, %s);", lv(j).c_str(), pv(j ^ m).c_str()));
+      } break;
+      case GN_YY: {
+        const int m = (1 << s0) | (1 << s1);
+        for (int j = 0; j < nreg; ++j) {
+          const int par = ((j >> s0) ^ (j >> s1)) & 1;
+          line(fmt("ga %s= imcj(%s, %s);", par ? "+" : "-", lv(j).c_str(), pv(j ^ m).c_str()));
+        }
+      } break;
+      case GN_ZZ:
+        for (int j = 0; j < nreg; ++j) {
+          const int par = ((j >> s0) ^ (j >> s1)) & 1;
+          line(fmt("ga %s= imcj(%s, %s);", par ? "-" : "+", lv(j).c_str(), pv(j).c_str()));
+        }
+        break;
+      case GN_M1: {
+        const int m = 1 << s0;
+        std::string g[8];
+        for (int i = 0; i < 8; ++i) g[i] = fmt("ct[%d]", goff + i);
+        for (int j = 0; j < nreg; ++j) {
+          if (j & m) continue;
+          line(fmt("{ const cT x = %s, y = %s;", pv(j).c_str(), pv(j | m).c_str()));
+          line(fmt("  const cT u = cT{%s * x.x - %s * x.y + %s * y.x - %s * y.y, %s * x.y + %s * x.x + %s * y.y + %s * y.x};",
+                   g[0].c_str(), g[1].c_str(), g[2].c_str(), g[3].c_str(), g[0].c_str(), g[1].c_str(), g[2].c_str(),
+                   g[3].c_str()));
+          line(fmt("  const cT w = cT{%s * x.x - %s * x.y + %s * y.x - %s * y.y, %s * x.y + %s * x.x + %s * y.y + %s * y.x};",
+                   g[4].c_str(), g[5].c_str(), g[6].c_str(), g[7].c_str(), g[4].c_str(), g[5].c_str(), g[6].c_str(),
+                   g[7].c_str()));
+          line(fmt("  ga += imcj(%s, u) + imcj(%s, w); }", lv(j).c_str(), lv(j | m).c_str()));
+        }
+      } break;
+      case GN_M2: {
+        const int m0 = 1 << s0, m1 = 1 << s1;
+        for (int j = 0; j < nreg; ++j) {
+          if (j & (m0 | m1)) continue;
+          const int idx[4] = {j, j | m1, j | m0, j | m0 | m1};
+          line("{");
+          for (int r = 0; r < 4; ++r) {
+            std::string re, im;
+            for (int c = 0; c < 4; ++c) {
+              const int e = r * 4 + c;
+              re += fmt("%sct[%d] * %s.x - ct[%d] * %s.y", c ? " + " : "", goff + 2 * e, pv(idx[c]).c_str(),
+                        goff + 2 * e + 1, pv(idx[c]).c_str());
+              im += fmt("%sct[%d] * %s.y + ct[%d] * %s.x", c ? " + " : "", goff + 2 * e, pv(idx[c]).c_str(),
+                        goff + 2 * e + 1, pv(idx[c]).c_str());
+            }
+            line(fmt("  ga += imcj(%s, cT{%s, %s});", lv(idx[r]).c_str(), re.c_str(), im.c_str()));
+          }
+          line("}");
+        }
+      } break;
+      default:
+        return false;
+    }
+    return true;
+  }
+};
+
+bool pass_supported(const Sched& S, const DPass& dp) {
+  if (dp.plain_op >= 0 || dp.k <= 0) return false;
+  if ((1 << dp.tthreads_log) < 64 || dp.tthreads_log > 10) return false;
+  for (int g = 0; g < dp.ngroups; ++g) {
+    const DGroup& dg = S.dgroup[dp.group0 + g];
+    if (dg.lds_op) return false;
+    for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) {
+      const int k = S.dop[o].kind;
+      if (k == OK_UK || (k != OK_DG && (S.dop[o].s0 < 0))) return false;
+    }
+  }
+  return true;
+}
+
+// body of one pass kernel (everything after the signature); mbase is the
+// smallest per-item table offset used by the pass
+std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt, int* mbase_out) {
+  const DPass& dp = S.dpass[pi];
+  const HPass& hp = S.passes[pi];
+  Gen G(S, dp, hp, adj);
+  const int R = S.R, nreg = 1 << R;
+  const int k = dp.k, tl = dp.tthreads_log, NT = 1 << tl, nbl = dp.nblk_log;
+  const bool wide = P.n > 31;
+  const uint32_t cb = dt == 0 ? 8u : 16u;  // bytes per complex amplitude
+  const char* IT = wide ? "unsigned long long" : "unsigned";
+  int mb = -1;
+  for (int g = 0; g < dp.ngroups; ++g) {
+    const DGroup& dg = S.dgroup[dp.group0 + g];
+    for (int o = dg.op0; o < dg.op0 + dg.nops; ++o)
+      if (S.dop[o].moff >= 0 && (S.dop[o].kind != OK_X1 && S.dop[o].kind != OK_CX && S.dop[o].kind != OK_CZ &&
+                                 S.dop[o].kind != OK_SWAP))
+        mb = mb < 0 ? S.dop[o].moff : std::min(mb, S.dop[o].moff);
+  }
+  if (mb < 0) mb = 0;
+  G.mbase = mb;
+  *mbase_out = mb;
+
+  // block base and thread / element deposits
+  std::vector<int> bs, bd, ts, td;
+  for (int i = 0; i < nbl; ++i) {
+    bs.push_back(i);
+    bd.push_back(dp.blk_bits[i]);
+  }
+  for (int t = 0; t < tl; ++t) {
+    ts.push_back(t);
+    td.push_back(hp.tile[t]);
+  }
+  auto dep_const = [&](uint64_t a) {
+    uint64_t o = 0;
+    for (int t = 0; t < k; ++t)
+      if (a >> t & 1) o |= 1ull << hp.tile[t];
+    return o;
+  };
+  G.line("const unsigned tid = threadIdx.x;");
+  G.line(fmt("const unsigned long long blk = blockIdx.x & 0x%llxull;", (nbl >= 64) ? ~0ull : ((1ull << nbl) - 1)));
+  G.line(fmt("const unsigned long long b = (unsigned long long)blockIdx.x >> %d;", nbl));
+  G.line(fmt("const unsigned long long base = %s;", deposit_expr("blk", bs, bd, "unsigned long long").c_str()));
+  G.line(fmt("%s dlo = %s;", IT, deposit_expr("tid", ts, td, IT).c_str()));
+  G.line("unsigned slo = swz(tid) * (unsigned)sizeof(cT);");
+  G.line("const T* __restrict__ mt = mtab + b * mstride + mbase;");
+  const int nit = 1 << (k - tl);  // elements per thread
+  auto hi = [&](int it) { return (unsigned long long)dep_const((uint64_t)it << tl); };
+
+  if (!adj) {
+    G.line(fmt("__shared__ cT tile[%d];", 1 << k));
+    G.line("cT* __restrict__ s = st + b * sstride + base;");
+    for (int it = 0; it < nit; ++it) G.line(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
+    for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tile, slo ^ %uu) = r%d;", swz_h((uint32_t)it << tl) * cb, it));
+  } else {
+    G.line(fmt("__shared__ cT tp[%d];", 1 << k));
+    G.line(fmt("__shared__ cT tlm[%d];", 1 << k));
+    G.line(fmt("__shared__ double red[%d];", std::max(1, dp.ngop) * (NT / 64)));
+    G.line("const unsigned lane = tid & 63u, wv = tid >> 6;");
+    G.line("{ const cT* sp = psi_in + b * sstride + base;");
+    G.line("  const cT* sl = lam_in + b * sstride + base;");
+    for (int it = 0; it < nit; ++it)
+      G.line(fmt("  const cT rp%d = sp[dlo + %lluull], rl%d = sl[dlo + %lluull];", it, hi(it), it, hi(it)));
+    for (int it = 0; it < nit; ++it)
+      G.line(fmt("  LDSB(tp, slo ^ %uu) = rp%d; LDSB(tlm, slo ^ %uu) = rl%d;", swz_h((uint32_t)it << tl) * cb, it,
+                 swz_h((uint32_t)it << tl) * cb, it));
+    G.line("}");
+  }
+  G.line("__syncthreads();");
+
+  int opid = 0;
+  const int ng = dp.ngroups;
+  for (int gg = 0; gg < ng; ++gg) {
+    const int g = adj ? ng - 1 - gg : gg;
+    const DGroup& dg = S.dgroup[dp.group0 + g];
+    G.where.assign(nreg, 0);
+    for (int j = 0; j < nreg; ++j) G.where[j] = j;
+    std::vector<int> cs, cd;
+    for (int r = 0; r < dg.nrun; ++r) {
+      const int w = dg.run[r];
+      const int src = w & 0xFF, len = (w >> 8) & 0xFF, dst = w >> 16;
+      for (int i = 0; i < len; ++i) {
+        cs.push_back(src + i);
+        cd.push_back(dst + i);
+      }
+    }
+    G.line("{");
+    G.line(fmt("const unsigned a0 = swz(%s) * (unsigned)sizeof(cT);", deposit_expr("tid", cs, cd, "unsigned").c_str()));
+    if (!adj) {
+      for (int j = 0; j < nreg; ++j) G.line(fmt("cT v%d = LDSB(tile, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb));
+      for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) G.apply(S.dop[o], {"v"}, opid++);
+      G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
+      for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tile, a1 ^ %uu) = v%d;", (uint32_t)dg.S[j] * cb, G.where[j]));
+    } else {
+      for (int j = 0; j < nreg; ++j)
+        G.line(fmt("cT p%d = LDSB(tp, a0 ^ %uu), l%d = LDSB(tlm, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb, j,
+                   (uint32_t)dg.S[j] * cb));
+      for (int o = dg.op0 + dg.nops - 1; o >= dg.op0; --o) {
+        const DOp& op = S.dop[o];
+        if (op.kind == OK_DG && op.nmem > 0) {
+          G.line("{");
+          for (int j = 0; j < nreg; ++j) G.line(fmt("const T im%d = imcj(%s, %s);", j, G.lv(j).c_str(), G.pv(j).c_str()));
+          for (int q = 0; q < op.nmem; ++q) {
+            const DMember& M = S.dmem[op.mem0 + q];
+            std::string e;
+            for (int j = 0; j < nreg; ++j) {
+              int sg = 0;
+              if (M.gen == GN_Z) sg = (j >> M.s0 & 1) ? -1 : 1;
+              else if (M.gen == GN_P1) sg = (j >> M.s0 & 1) ? 1 : 0;
+              else if (M.gen == GN_ZZ) sg = (((j >> M.s0) ^ (j >> M.s1)) & 1) ? -1 : 1;
+              if (sg) e += fmt(" %s im%d", sg > 0 ? "+" : "-", j);
+            }
+            if (e.empty()) e = " + 0";
+            G.line(fmt("{ const T ga = (T)0%s; const double gs = wsum((double)ga);", e.c_str()));
+            G.line(fmt("  if (lane == 0) red[%d + wv] = gs; }", M.gloc * (NT / 64)));
+          }
+          G.line("}");
+        } else if (op.gen != GN_NONE) {
+          G.line("{");
+          if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return std::string();
+          G.line("const double gs = wsum((double)ga);");
+          G.line(fmt("if (lane == 0) red[%d + wv] = gs;", op.gloc * (NT / 64)));
+          G.line("}");
+        }
+        G.apply(op, {"p", "l"}, opid++);
+      }
+      G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
+      for (int j = 0; j < nreg; ++j)
+        G.line(fmt("LDSB(tp, a1 ^ %uu) = p%d; LDSB(tlm, a1 ^ %uu) = l%d;", (uint32_t)dg.S[j] * cb, G.where[j],
+                   (uint32_t)dg.S[j] * cb, G.where[j]));
+    }
+    G.line("}");
+    G.line("__syncthreads();");
+  }
+
+  // recompute the store addresses rather than holding the load addresses live
+  G.line("asm volatile(\"\" : \"+v\"(dlo), \"+v\"(slo));");
+  if (!adj) {
+    for (int it = 0; it < nit; ++it) {
+      G.line(fmt("{ const cT x = LDSB(tile, slo ^ %uu); s[dlo + %lluull] = x;", swz_h((uint32_t)it << tl) * cb, hi(it)));
+      G.line(fmt("  if (ck) ck[b * sstride + base + dlo + %lluull] = x; }", hi(it)));
+    }
+  } else {
+    G.line("{ cT* sl = lam_out + b * sstride + base;");
+    G.line("  if (psi_out) { cT* sp = psi_out + b * sstride + base;");
+    for (int it = 0; it < nit; ++it)
+      G.line(fmt("    sp[dlo + %lluull] = LDSB(tp, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
+    G.line("  }");
+    for (int it = 0; it < nit; ++it)
+      G.line(fmt("  sl[dlo + %lluull] = LDSB(tlm, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
+    G.line("}");
+    if (dp.ngop > 0) {
+      G.line(fmt("for (unsigned q = tid; q < %du; q += %du) {", dp.ngop, NT));
+      G.line("  double sum = 0.0;");
+      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += red[q * %d + w];", NT / 64, NT / 64));
+      G.line(fmt("  part[b * pstride + (long long)q * %lldll + (long long)blk] = sum;", 1ll << nbl));
+      G.line("}");
+    }
+  }
+  return G.out;
+}
+
+}  // namespace
+
+std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, std::vector<JitPass>& jp,
+                            std::vector<std::string>& bodies) {
+  jp.assign(S.dpass.size(), JitPass{});
+  std::unordered_map<std::string, int> seen;
+  std::string src;
+  bodies.clear();
+  for (int pi = 0; pi < (int)S.dpass.size(); ++pi) {
+    const DPass& dp = S.dpass[pi];
+    if (!pass_supported(S, dp)) continue;
+    int mb = 0;
+    std::string body = pass_body(P, S, pi, mode == 1, dt, &mb);
+    if (body.empty()) continue;
+    const int NT = 1 << dp.tthreads_log;
+    const std::string key = fmt("%d|", NT) + body;
+    auto it = seen.find(key);
+    int id;
+    if (it == seen.end()) {
+      id = (int)bodies.size();
+      seen.emplace(key, id);
+      std::string k;
+      if (mode == 0)
+        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_k%d(cT* __restrict__ st, long long sstride, "
+                "const T* __restrict__ mtab, long long mstride, long long mbase, const T* __restrict__ ct, "
+                "cT* __restrict__ ck) {\n",
+                NT, id);
+      else
+        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_k%d(const cT* psi_in, "
+                "cT* psi_out, const cT* lam_in, cT* lam_out, "
+                "long long sstride, const T* __restrict__ mtab, long long mstride, long long mbase, "
+                "const T* __restrict__ ct, double* __restrict__ part, long long pstride) {\n",
+                NT, id);
+      bodies.push_back(k + body + "}\n");
+    } else {
+      id = it->second;
+    }
+    jp[pi].kernel = id;
+    jp[pi].mbase = mb;
+  }
+  src = std::string("#define REAL ") + (dt == 0 ? "float" : "double") + "\n" + kPrelude;
+  return src;
+}
+
+}  // namespace tcm

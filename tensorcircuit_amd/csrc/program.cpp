@@ -703,12 +703,14 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
     return rc;
   }
   const tcm_opts& o = P->opts;
-  if (o.reg_slots != 0 && o.reg_slots != 4) {
+  // generic kernels are built for 4 register-resident qubits; compiled
+  // passes take 3 or 4
+  if (o.reg_slots != 0 && o.reg_slots != 4 && !(o.jit && o.reg_slots == 3)) {
     delete P;
-    return set_error(TCM_E_UNSUPPORTED, "reg_slots=%d: only 4 register-resident qubits are built", o.reg_slots);
+    return set_error(TCM_E_UNSUPPORTED, "reg_slots=%d: 4 register-resident qubits (3 or 4 with jit)", o.reg_slots);
   }
   for (int dt = 0; dt < 2; ++dt) {
-    const int R = 4;
+    const int R = o.reg_slots ? o.reg_slots : 4;
     const int Lw = dt == 0 ? 4 : 3;
     // default tiles keep one array at 64 KiB of LDS (forward) and two
     // arrays at 64 KiB together (adjoint)

@@ -159,6 +159,13 @@ struct HPass {
 
 struct DeviceTables;  // per-device uploaded tables (kernels.hip)
 
+// Compiled pass kernels (pass compiler): kernel index per pass (-1 = the
+// generic fused kernel runs it) and the pass's per-item table offset base.
+struct JitPass {
+  int32_t kernel = -1;
+  int32_t mbase = 0;
+};
+
 // One schedule: a sequence of passes for one dtype and one mode (forward /
 // adjoint), plus the flattened device descriptors and matrix recipes.
 struct Sched {
@@ -182,6 +189,12 @@ struct Sched {
   std::vector<int32_t> gop_slots;  // per gop: number of partial slots (blocks)
   std::vector<int32_t> gop_list;   // per pass (from DPass::gop_list0): gops by gloc
   std::vector<DeviceTables*> dev;  // per device
+  // pass compiler output (opts.jit): 0 not built, 1 built, -1 failed
+  int32_t jit_state = 0;
+  std::vector<JitPass> jit;
+  std::vector<std::vector<char>> jit_code;  // code objects, one per chunk
+  std::vector<int32_t> jit_chunk;           // kernel -> chunk
+  double jit_seconds = 0.0;                 // compile time (0 when cached)
 };
 
 struct Program {
@@ -199,6 +212,11 @@ struct Program {
   Sched sched[2][2];             // [dtype][mode 0 fwd / 1 adjoint]
   std::mutex mu;
 };
+
+// pass compiler (passgen.cpp, jit.cpp)
+std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, std::vector<JitPass>& jp,
+                            std::vector<std::string>& bodies);
+int jit_build(const Program& P, Sched& S, int dt, int mode);
 
 // host scheduling (program.cpp)
 int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t nparams,

@@ -80,7 +80,7 @@ class Program:
     fused passes (1) or one kernel per gate (0, the plain path)."""
 
     def __init__(self, n: int, specs: Sequence[GateSpec], nparams: int, fuse: int = 1, tile_bits: int = 0,
-                 adj_tile_bits: int = 0, ckpt_every: int = 0, reg_slots: int = 0, mfma: int = 0):
+                 adj_tile_bits: int = 0, ckpt_every: int = 0, reg_slots: int = 0, mfma: int = 0, jit: int = 0):
         self.n = int(n)
         self.nparams = int(nparams)
         self.specs = list(specs)
@@ -100,7 +100,7 @@ class Program:
                 arr[i].mat = m.ctypes.data_as(C.POINTER(C.c_double))
         opts = _lib.TcmOpts()
         opts.fuse, opts.tile_bits, opts.adj_tile_bits = fuse, tile_bits, adj_tile_bits
-        opts.ckpt_every, opts.reg_slots, opts.mfma = ckpt_every, reg_slots, mfma
+        opts.ckpt_every, opts.reg_slots, opts.mfma, opts.jit = ckpt_every, reg_slots, mfma, jit
         h = C.c_void_p()
         check(lib.tcm_program_build(arr, len(self.specs), self.n, self.nparams, C.byref(opts), C.byref(h)),
               "tcm_program_build")
@@ -118,6 +118,13 @@ class Program:
         if h:
             _free(h)
             self.handle = None
+
+    def compile(self, dtype: str = "complex64", mode: int = 0) -> float:
+        """Compile the pass kernels of one schedule now (programs built with
+        ``jit=1``); returns the compile seconds (0.0 when cached)."""
+        sec = C.c_double(0.0)
+        check(lib.tcm_program_compile(self.handle, _CODE[dtype], int(mode), C.byref(sec)), "tcm_program_compile")
+        return sec.value
 
     def describe(self, dtype: str = "complex64", mode: int = 0) -> Dict:
         n = lib.tcm_program_describe(self.handle, _CODE[dtype], mode, None, 0)
