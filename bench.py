@@ -23,11 +23,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_fused_adj launch measured with rocprofv3 --pmc FETCH_SIZE /
-# WRITE_SIZE on the default workload (24q, 20 layers, batch 32, 11-bit adjoint
-# tiles), FETCH scaled x2.0 and WRITE x1.0 by a 4 GiB copy of the same run;
-# the commands and per-counter figures are in profiles/k_fused_adj_hea24_b32.md.
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5): 17046440436}
+# HBM bytes per adjoint pass launch measured with rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE on the default workload (24q, 20 layers, batch 32, checkpoints
+# every 5 layers, default tiles), FETCH scaled x2.0 and WRITE x1.0 by a 4 GiB
+# copy in the same run.  Key: (qubits, layers, batch, ckpt, jit).  Commands
+# and per-counter figures: profiles/tcm_adj_passes_hea24_b32.md (jit=1) and
+# profiles/k_fused_adj_hea24_b32.md (jit=0).
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0): 17046440436, (24, 20, 32, 5, 1): 16945769829}
 
 
 def parse():
@@ -41,7 +43,7 @@ def parse():
     ap.add_argument("--ckpt", type=int, default=5)
     ap.add_argument("--tile", type=int, default=0, help="forward tile bits (0 = default)")
     ap.add_argument("--adj-tile", type=int, default=0, help="adjoint tile bits (0 = default)")
-    ap.add_argument("--jit", type=int, default=0, help="1: compiled pass kernels (pass compiler)")
+    ap.add_argument("--jit", type=int, default=1, help="1: compiled pass kernels (pass compiler), 0: generic")
     ap.add_argument("--reg-slots", type=int, default=0, help="register-resident qubits (0 = default)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -197,13 +199,15 @@ def main():
         adj_gbs = adj_bytes / (adj_launch_ms * 1e-3) / 1e9
         fwd_gbs = fwd_bytes / (fwd_launch_ms * 1e-3) / 1e9
         traffic = None
-        if args.adj_tile in (0, 11):
-            traffic = MEASURED_ADJ_TRAFFIC.get((n, L, B, args.ckpt))
-        out["roofline"] = {"bound": "hbm", "kernel": "k_fused_adj", "achieved": round(adj_gbs, 1),
+        if args.adj_tile == 0 and args.reg_slots in (0, 4):
+            traffic = MEASURED_ADJ_TRAFFIC.get((n, L, B, args.ckpt, args.jit))
+        kname = "tcm_adj_k* (compiled adjoint passes)" if args.jit else "k_fused_adj"
+        out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": round(adj_gbs, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
                            "traffic": traffic, "bytes_per_launch": adj_bytes,
                            "avg_launch_ms": round(adj_launch_ms, 4), "launches_per_step": n_adj}
-        out["roofline_fwd"] = {"kernel": "k_fused_fwd", "achieved": round(fwd_gbs, 1), "unit": "GB/s",
+        out["roofline_fwd"] = {"kernel": "tcm_fwd_k* (compiled forward passes)" if args.jit else "k_fused_fwd",
+                               "achieved": round(fwd_gbs, 1), "unit": "GB/s",
                                "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": fwd_bytes,
                                "avg_launch_ms": round(fwd_launch_ms, 4), "launches_per_step": n_fwd}
         print("fused fwd pass %.3f ms (%.0f GB/s), adjoint pass %.3f ms (%.0f GB/s)"

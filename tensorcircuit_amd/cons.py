@@ -1,12 +1,32 @@
 """Global configuration (the reference's cons.py:28-222 subset for this path):
-the complex dtype and the backend handle."""
+the complex dtype, the backend handle and the pass-compiler policy."""
 from __future__ import annotations
 
 import contextlib
+import os
 from typing import Iterator, Optional
 
 dtypestr = "complex64"
 rdtypestr = "float32"
+
+# Pass compiler policy for programs built by the Circuit API: None = compile
+# the tile passes of states with at least JIT_MIN_QUBITS qubits (their
+# passes run for milliseconds, so the one-off compile is amortised and
+# cached on disk), True / False = always / never.  TCM_JIT=0/1 sets it.
+jit: Optional[bool] = {"0": False, "1": True}.get(os.environ.get("TCM_JIT", ""), None)
+JIT_MIN_QUBITS = 20
+
+
+def set_jit(flag: Optional[bool]) -> None:
+    """Choose the pass-compiler policy (None = by state size)."""
+    global jit
+    jit = flag
+
+
+def jit_for(nqubits: int) -> int:
+    if jit is None:
+        return 1 if nqubits >= JIT_MIN_QUBITS else 0
+    return 1 if jit else 0
 
 
 def set_dtype(dtype: Optional[str] = None) -> str:

@@ -573,6 +573,8 @@ int forward_multi_impl(const tcm_multi_desc* descs, int ndesc, int dtype, int in
     Program& P = descs[i].prog->p;
     if (int rc = ensure_tables(P, dtype, 0, &D[i])) return rc;
     const Sched& S = P.sched[dtype][0];
+    if (S.k > 0 && S.R != 4)
+      return set_error(TCM_E_UNSUPPORTED, "multi-program launches run 4 register slots (program %d has %d)", i, S.R);
     mtab[i] = cv.take(descs[i].batch * (int64_t)std::max(S.mtab_size, 1) * sizeof(T));
     if (descs[i].batch <= 0) continue;
     launch_prep<T>(S, *D[i], D[i]->ctab, descs[i].params, descs[i].param_stride, mtab[i], descs[i].batch, st);

@@ -114,6 +114,7 @@ void rtc_compile(const std::string& src, const std::vector<std::string>& opts, s
 
 int jit_build(const Program& P, Sched& S, int dt, int mode) {
   const auto t0 = std::chrono::steady_clock::now();
+  S.jit_prefix = mode == 0 ? "tcm_fwd_k" : "tcm_adj_k";
   std::vector<std::string> bodies;
   const std::string prelude = gen_pass_source(P, S, dt, mode, S.jit, bodies);
   if (bodies.empty()) {
@@ -182,7 +183,7 @@ int jit_load(const Sched& S, DeviceTables& D) {
   D.jit_fn.assign(S.jit_chunk.size(), nullptr);
   for (size_t k = 0; k < S.jit_chunk.size(); ++k) {
     hipFunction_t f;
-    const std::string name = "tcm_k" + std::to_string(k);
+    const std::string name = S.jit_prefix + std::to_string(k);
     const hipError_t e = hipModuleGetFunction(&f, mods[S.jit_chunk[k]], name.c_str());
     if (e != hipSuccess) return set_error((int)e, "hipModuleGetFunction(%s): %s", name.c_str(), hipGetErrorString(e));
     D.jit_fn[k] = (void*)f;

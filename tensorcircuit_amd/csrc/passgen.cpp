@@ -562,12 +562,12 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
       seen.emplace(key, id);
       std::string k;
       if (mode == 0)
-        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_k%d(cT* __restrict__ st, long long sstride, "
+        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_fwd_k%d(cT* __restrict__ st, long long sstride, "
                 "const T* __restrict__ mtab, long long mstride, long long mbase, const T* __restrict__ ct, "
                 "cT* __restrict__ ck) {\n",
                 NT, id);
       else
-        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_k%d(const cT* psi_in, "
+        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_adj_k%d(const cT* psi_in, "
                 "cT* psi_out, const cT* lam_in, cT* lam_out, "
                 "long long sstride, const T* __restrict__ mtab, long long mstride, long long mbase, "
                 "const T* __restrict__ ct, double* __restrict__ part, long long pstride) {\n",

@@ -705,19 +705,21 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
   const tcm_opts& o = P->opts;
   // generic kernels are built for 4 register-resident qubits; compiled
   // passes take 3 or 4
-  if (o.reg_slots != 0 && o.reg_slots != 4 && !(o.jit && o.reg_slots == 3)) {
+  if (o.reg_slots != 0 && o.reg_slots != 4 && !(o.jit && o.reg_slots >= 3 && o.reg_slots <= 5)) {
     delete P;
-    return set_error(TCM_E_UNSUPPORTED, "reg_slots=%d: 4 register-resident qubits (3 or 4 with jit)", o.reg_slots);
+    return set_error(TCM_E_UNSUPPORTED, "reg_slots=%d: 4 register-resident qubits (3 to 5 with jit)", o.reg_slots);
   }
   for (int dt = 0; dt < 2; ++dt) {
     const int R = o.reg_slots ? o.reg_slots : 4;
     const int Lw = dt == 0 ? 4 : 3;
-    // default tiles keep one array at 64 KiB of LDS (forward) and two
-    // arrays at 64 KiB together (adjoint)
     // defaults measured on MI355X: 32 KiB forward tiles (several blocks per
-    // CU overlap HBM and compute); the adjoint holds two tiles
-    int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 12 : 11);
-    int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (dt == 0 ? 11 : 10);
+    // CU overlap HBM and compute); the adjoint holds two tiles.  Compiled
+    // passes (jit) hold fewer registers and take one size larger tiles:
+    // fewer passes at about the same per-pass bandwidth (24-qubit HEA:
+    // 44 + 56 passes instead of 56 + 68)
+    const int kbig = o.jit ? 1 : 0;
+    int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 12 : 11) + kbig;
+    int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (dt == 0 ? 11 : 10) + kbig;
     // fused passes need at least one full wave (64 threads) per tile
     bool fused = o.fuse != 0 && n >= R + 6;
     for (int mode = 0; mode < 2; ++mode) {

@@ -195,6 +195,7 @@ struct Sched {
   std::vector<std::vector<char>> jit_code;  // code objects, one per chunk
   std::vector<int32_t> jit_chunk;           // kernel -> chunk
   double jit_seconds = 0.0;                 // compile time (0 when cached)
+  std::string jit_prefix;                   // kernel names: prefix + index
 };
 
 struct Program {

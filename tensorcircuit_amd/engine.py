@@ -306,6 +306,10 @@ def get_program(n: int, specs: Sequence[GateSpec], nparams: int, **opts: Any) ->
     """Build-once cache keyed by structure (gate kinds, qubits, constant
     matrices, parameter slots) and options -- the reference rebuilds its
     tensor network on every call (basecircuit.py:120-245)."""
+    if "jit" not in opts and opts.get("fuse", 1):
+        from . import cons
+
+        opts["jit"] = cons.jit_for(n)
     key = (n, nparams, tuple(s.key() + (s.param,) for s in specs), tuple(sorted(opts.items())))
     with _PROG_LOCK:
         p = _PROG_CACHE.get(key)
