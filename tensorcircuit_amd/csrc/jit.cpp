@@ -5,6 +5,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -201,8 +202,10 @@ int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const 
   const void* a_mt = mtab;
   const void* a_ct = ctab;
   void* a_ck = ckout;
-  void* args[] = {&a_st, &sstride, &a_mt, &mstride, &mbase, &a_ct, &a_ck};
-  const unsigned blocks = (unsigned)(batch << P.nblk_log);
+  long long total = batch << P.nblk_log;
+  // one entry per parameter of the generated signature, in its order
+  void* args[] = {&a_st, &sstride, &a_mt, &mstride, &mbase, &a_ct, &a_ck, &total};
+  const unsigned blocks = (unsigned)total;
   const hipError_t e = hipModuleLaunchKernel((hipFunction_t)D.jit_fn[J.kernel], blocks, 1, 1,
                                              1u << P.tthreads_log, 1, 1, 0, st, args, nullptr);
   if (e != hipSuccess) return set_error((int)e, "hipModuleLaunchKernel: %s", hipGetErrorString(e));
@@ -223,8 +226,9 @@ int jit_launch_adj(const Sched& S, const DeviceTables& D, int pi, int dt, const 
   const void* a_mt = mtab;
   const void* a_ct = ctab;
   double* a_part = part;
-  void* args[] = {&a_pi, &a_po, &a_li, &a_lo, &sstride, &a_mt, &mstride, &mbase, &a_ct, &a_part, &ps};
-  const unsigned blocks = (unsigned)(batch << P.nblk_log);
+  long long total = batch << P.nblk_log;
+  void* args[] = {&a_pi, &a_po, &a_li, &a_lo, &sstride, &a_mt, &mstride, &mbase, &a_ct, &a_part, &ps, &total};
+  const unsigned blocks = (unsigned)total;
   const hipError_t e = hipModuleLaunchKernel((hipFunction_t)D.jit_fn[J.kernel], blocks, 1, 1,
                                              1u << P.tthreads_log, 1, 1, 0, st, args, nullptr);
   if (e != hipSuccess) return set_error((int)e, "hipModuleLaunchKernel: %s", hipGetErrorString(e));

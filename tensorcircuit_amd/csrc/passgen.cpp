@@ -14,6 +14,7 @@
 // launcher passes as `mbase`.
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -416,6 +417,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
       if (a >> t & 1) o |= 1ull << hp.tile[t];
     return o;
   };
+  G.line("if ((unsigned long long)blockIdx.x >= (unsigned long long)total) return;");
   G.line("const unsigned tid = threadIdx.x;");
   G.line(fmt("const unsigned long long blk = blockIdx.x & 0x%llxull;", (nbl >= 64) ? ~0ull : ((1ull << nbl) - 1)));
   G.line(fmt("const unsigned long long b = (unsigned long long)blockIdx.x >> %d;", nbl));
@@ -426,54 +428,10 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   const int nit = 1 << (k - tl);  // elements per thread
   auto hi = [&](int it) { return (unsigned long long)dep_const((uint64_t)it << tl); };
 
-  if (!adj) {
-    G.line(fmt("__shared__ cT tile[%d];", 1 << k));
-    G.line("cT* __restrict__ s = st + b * sstride + base;");
-    for (int it = 0; it < nit; ++it) G.line(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
-    for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tile, slo ^ %uu) = r%d;", swz_h((uint32_t)it << tl) * cb, it));
-  } else {
-    G.line(fmt("__shared__ cT tp[%d];", 1 << k));
-    G.line(fmt("__shared__ cT tlm[%d];", 1 << k));
-    G.line(fmt("__shared__ double red[%d];", std::max(1, dp.ngop) * (NT / 64)));
-    G.line("const unsigned lane = tid & 63u, wv = tid >> 6;");
-    G.line("{ const cT* sp = psi_in + b * sstride + base;");
-    G.line("  const cT* sl = lam_in + b * sstride + base;");
-    for (int it = 0; it < nit; ++it)
-      G.line(fmt("  const cT rp%d = sp[dlo + %lluull], rl%d = sl[dlo + %lluull];", it, hi(it), it, hi(it)));
-    for (int it = 0; it < nit; ++it)
-      G.line(fmt("  LDSB(tp, slo ^ %uu) = rp%d; LDSB(tlm, slo ^ %uu) = rl%d;", swz_h((uint32_t)it << tl) * cb, it,
-                 swz_h((uint32_t)it << tl) * cb, it));
-    G.line("}");
-  }
-  G.line("__syncthreads();");
-
   int opid = 0;
-  const int ng = dp.ngroups;
-  for (int gg = 0; gg < ng; ++gg) {
-    const int g = adj ? ng - 1 - gg : gg;
-    const DGroup& dg = S.dgroup[dp.group0 + g];
-    G.where.assign(nreg, 0);
-    for (int j = 0; j < nreg; ++j) G.where[j] = j;
-    std::vector<int> cs, cd;
-    for (int r = 0; r < dg.nrun; ++r) {
-      const int w = dg.run[r];
-      const int src = w & 0xFF, len = (w >> 8) & 0xFF, dst = w >> 16;
-      for (int i = 0; i < len; ++i) {
-        cs.push_back(src + i);
-        cd.push_back(dst + i);
-      }
-    }
-    G.line("{");
-    G.line(fmt("const unsigned a0 = swz(%s) * (unsigned)sizeof(cT);", deposit_expr("tid", cs, cd, "unsigned").c_str()));
-    if (!adj) {
-      for (int j = 0; j < nreg; ++j) G.line(fmt("cT v%d = LDSB(tile, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb));
-      for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) G.apply(S.dop[o], {"v"}, opid++);
-      G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
-      for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tile, a1 ^ %uu) = v%d;", (uint32_t)dg.S[j] * cb, G.where[j]));
-    } else {
-      for (int j = 0; j < nreg; ++j)
-        G.line(fmt("cT p%d = LDSB(tp, a0 ^ %uu), l%d = LDSB(tlm, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb, j,
-                   (uint32_t)dg.S[j] * cb));
+  // gradients and inverse gates of one group, in reverse program order, on
+  // the register arrays p (psi) and l (lambda)
+  auto emit_adj_ops = [&](const DGroup& dg) -> bool {
       for (int o = dg.op0 + dg.nops - 1; o >= dg.op0; --o) {
         const DOp& op = S.dop[o];
         if (op.kind == OK_DG && op.nmem > 0) {
@@ -496,37 +454,99 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
           G.line("}");
         } else if (op.gen != GN_NONE) {
           G.line("{");
-          if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return std::string();
+          if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return false;
           G.line("const double gs = wsum((double)ga);");
           G.line(fmt("if (lane == 0) red[%d + wv] = gs;", op.gloc * (NT / 64)));
           G.line("}");
         }
         G.apply(op, {"p", "l"}, opid++);
       }
-      G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
-      for (int j = 0; j < nreg; ++j)
-        G.line(fmt("LDSB(tp, a1 ^ %uu) = p%d; LDSB(tlm, a1 ^ %uu) = l%d;", (uint32_t)dg.S[j] * cb, G.where[j],
-                   (uint32_t)dg.S[j] * cb, G.where[j]));
+    return true;
+  };
+  if (adj) {
+    // Adjoint pass.  One LDS tile shared by psi and lambda: each layout change moves psi
+    // through it, then lambda (four barriers instead of one), for half the
+    // LDS of two tiles and twice the resident blocks per CU.
+    G.line(fmt("__shared__ cT tb[%d];", 1 << k));
+    G.line(fmt("__shared__ double red[%d];", std::max(1, dp.ngop) * (NT / 64)));
+    G.line("const unsigned lane = tid & 63u, wv = tid >> 6;");
+    {
+      std::string decl = "cT";
+      for (int j = 0; j < nreg; ++j) decl += fmt("%s p%d, l%d", j ? "," : "", j, j);
+      G.line(decl + ";");
     }
+    G.line("const cT* sp = psi_in + b * sstride + base;");
+    G.line("const cT* sl = lam_in + b * sstride + base;");
+    for (int it = 0; it < nit; ++it) G.line(fmt("const cT rl%d = sl[dlo + %lluull];", it, hi(it)));
+    G.line("{");
+    for (int it = 0; it < nit; ++it) G.line(fmt("const cT rp%d = sp[dlo + %lluull];", it, hi(it)));
+    for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tb, slo ^ %uu) = rp%d;", swz_h((uint32_t)it << tl) * cb, it));
     G.line("}");
     G.line("__syncthreads();");
-  }
-
-  // recompute the store addresses rather than holding the load addresses live
-  G.line("asm volatile(\"\" : \"+v\"(dlo), \"+v\"(slo));");
-  if (!adj) {
-    for (int it = 0; it < nit; ++it) {
-      G.line(fmt("{ const cT x = LDSB(tile, slo ^ %uu); s[dlo + %lluull] = x;", swz_h((uint32_t)it << tl) * cb, hi(it)));
-      G.line(fmt("  if (ck) ck[b * sstride + base + dlo + %lluull] = x; }", hi(it)));
+    const int ng1 = dp.ngroups;
+    int prev = -1;
+    std::vector<int> prev_where;
+    auto group_addr = [&](const DGroup& dg, int g) {
+      std::vector<int> cs, cd;
+      for (int r = 0; r < dg.nrun; ++r) {
+        const int w = dg.run[r];
+        const int src = w & 0xFF, len = (w >> 8) & 0xFF, dst = w >> 16;
+        for (int i = 0; i < len; ++i) {
+          cs.push_back(src + i);
+          cd.push_back(dst + i);
+        }
+      }
+      G.line(fmt("const unsigned ga%d = swz(%s) * (unsigned)sizeof(cT);", g,
+                 deposit_expr("tid", cs, cd, "unsigned").c_str()));
+    };
+    auto write_regs = [&](const char* a, int g, const std::vector<int>& wh) {
+      const DGroup& dg = S.dgroup[dp.group0 + g];
+      for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tb, ga%d ^ %uu) = %s%d;", g, (uint32_t)dg.S[j] * cb, a, wh[j]));
+    };
+    auto read_regs = [&](const char* a, int g) {
+      const DGroup& dg = S.dgroup[dp.group0 + g];
+      for (int j = 0; j < nreg; ++j) G.line(fmt("%s%d = LDSB(tb, ga%d ^ %uu);", a, j, g, (uint32_t)dg.S[j] * cb));
+    };
+    for (int gg = 0; gg < ng1; ++gg) {
+      const int g = ng1 - 1 - gg;
+      const DGroup& dg = S.dgroup[dp.group0 + g];
+      group_addr(dg, g);
+      if (prev < 0) {
+        read_regs("p", g);
+        G.line("__syncthreads();");
+        for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tb, slo ^ %uu) = rl%d;", swz_h((uint32_t)it << tl) * cb, it));
+        G.line("__syncthreads();");
+        read_regs("l", g);
+      } else {
+        G.line("__syncthreads();");
+        write_regs("p", prev, prev_where);
+        G.line("__syncthreads();");
+        read_regs("p", g);
+        G.line("__syncthreads();");
+        write_regs("l", prev, prev_where);
+        G.line("__syncthreads();");
+        read_regs("l", g);
+      }
+      G.where.assign(nreg, 0);
+      for (int j = 0; j < nreg; ++j) G.where[j] = j;
+      if (!emit_adj_ops(dg)) return std::string();
+      prev = g;
+      prev_where = G.where;
     }
-  } else {
-    G.line("{ cT* sl = lam_out + b * sstride + base;");
-    G.line("  if (psi_out) { cT* sp = psi_out + b * sstride + base;");
+    G.line("asm volatile(\"\" : \"+v\"(dlo), \"+v\"(slo));");
+    G.line("__syncthreads();");
+    write_regs("p", prev, prev_where);
+    G.line("__syncthreads();");
+    G.line("if (psi_out) { cT* so = psi_out + b * sstride + base;");
     for (int it = 0; it < nit; ++it)
-      G.line(fmt("    sp[dlo + %lluull] = LDSB(tp, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
-    G.line("  }");
+      G.line(fmt("  so[dlo + %lluull] = LDSB(tb, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
+    G.line("}");
+    G.line("__syncthreads();");
+    write_regs("l", prev, prev_where);
+    G.line("__syncthreads();");
+    G.line("{ cT* so = lam_out + b * sstride + base;");
     for (int it = 0; it < nit; ++it)
-      G.line(fmt("  sl[dlo + %lluull] = LDSB(tlm, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
+      G.line(fmt("  so[dlo + %lluull] = LDSB(tb, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
     G.line("}");
     if (dp.ngop > 0) {
       G.line(fmt("for (unsigned q = tid; q < %du; q += %du) {", dp.ngop, NT));
@@ -535,6 +555,41 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
       G.line(fmt("  part[b * pstride + (long long)q * %lldll + (long long)blk] = sum;", 1ll << nbl));
       G.line("}");
     }
+    return G.out;
+  }
+  // forward pass: one tile
+  G.line(fmt("__shared__ cT tile[%d];", 1 << k));
+  G.line("cT* __restrict__ s = st + b * sstride + base;");
+  for (int it = 0; it < nit; ++it) G.line(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
+  for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tile, slo ^ %uu) = r%d;", swz_h((uint32_t)it << tl) * cb, it));
+  G.line("__syncthreads();");
+  for (int g = 0; g < dp.ngroups; ++g) {
+    const DGroup& dg = S.dgroup[dp.group0 + g];
+    G.where.assign(nreg, 0);
+    for (int j = 0; j < nreg; ++j) G.where[j] = j;
+    std::vector<int> cs, cd;
+    for (int r = 0; r < dg.nrun; ++r) {
+      const int w = dg.run[r];
+      const int src = w & 0xFF, len = (w >> 8) & 0xFF, dst = w >> 16;
+      for (int i = 0; i < len; ++i) {
+        cs.push_back(src + i);
+        cd.push_back(dst + i);
+      }
+    }
+    G.line("{");
+    G.line(fmt("const unsigned a0 = swz(%s) * (unsigned)sizeof(cT);", deposit_expr("tid", cs, cd, "unsigned").c_str()));
+    for (int j = 0; j < nreg; ++j) G.line(fmt("cT v%d = LDSB(tile, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb));
+    for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) G.apply(S.dop[o], {"v"}, opid++);
+    G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
+    for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tile, a1 ^ %uu) = v%d;", (uint32_t)dg.S[j] * cb, G.where[j]));
+    G.line("}");
+    G.line("__syncthreads();");
+  }
+  // recompute the store addresses rather than holding the load addresses live
+  G.line("asm volatile(\"\" : \"+v\"(dlo), \"+v\"(slo));");
+  for (int it = 0; it < nit; ++it) {
+    G.line(fmt("{ const cT x = LDSB(tile, slo ^ %uu); s[dlo + %lluull] = x;", swz_h((uint32_t)it << tl) * cb, hi(it)));
+    G.line(fmt("  if (ck) ck[b * sstride + base + dlo + %lluull] = x; }", hi(it)));
   }
   return G.out;
 }
@@ -561,17 +616,27 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
       id = (int)bodies.size();
       seen.emplace(key, id);
       std::string k;
+      // register target: as many waves per SIMD as the LDS tiles allow
+      // (160 KiB per CU), so registers never cut occupancy below that
+      const size_t cbytes = dt == 0 ? 8 : 16;
+      const size_t lds = (cbytes << dp.k) +
+                         (mode == 0 ? 0 : 8 * (size_t)std::max(1, dp.ngop) * (size_t)(NT / 64));
+      const int blocks_cu = (int)std::max<size_t>(1, (160u * 1024u) / lds);
+      const int waves = std::max(1, std::min(8, blocks_cu * (NT / 64) / 4));
+      // parameter lists: jit_launch_fwd / jit_launch_adj pass exactly these
       if (mode == 0)
-        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_fwd_k%d(cT* __restrict__ st, long long sstride, "
+        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "
+                "tcm_fwd_k%d(cT* __restrict__ st, long long sstride, "
                 "const T* __restrict__ mtab, long long mstride, long long mbase, const T* __restrict__ ct, "
-                "cT* __restrict__ ck) {\n",
-                NT, id);
+                "cT* __restrict__ ck, long long total) {\n",
+                NT, waves, id);
       else
-        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_adj_k%d(const cT* psi_in, "
+        k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "
+                "tcm_adj_k%d(const cT* psi_in, "
                 "cT* psi_out, const cT* lam_in, cT* lam_out, "
                 "long long sstride, const T* __restrict__ mtab, long long mstride, long long mbase, "
-                "const T* __restrict__ ct, double* __restrict__ part, long long pstride) {\n",
-                NT, id);
+                "const T* __restrict__ ct, double* __restrict__ part, long long pstride, long long total) {\n",
+                NT, waves, id);
       bodies.push_back(k + body + "}\n");
     } else {
       id = it->second;

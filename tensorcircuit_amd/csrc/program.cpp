@@ -713,13 +713,13 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
     const int R = o.reg_slots ? o.reg_slots : 4;
     const int Lw = dt == 0 ? 4 : 3;
     // defaults measured on MI355X: 32 KiB forward tiles (several blocks per
-    // CU overlap HBM and compute); the adjoint holds two tiles.  Compiled
-    // passes (jit) hold fewer registers and take one size larger tiles:
-    // fewer passes at about the same per-pass bandwidth (24-qubit HEA:
-    // 44 + 56 passes instead of 56 + 68)
-    const int kbig = o.jit ? 1 : 0;
-    int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 12 : 11) + kbig;
-    int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (dt == 0 ? 11 : 10) + kbig;
+    // CU overlap HBM and compute); the generic adjoint holds two tiles.
+    // Compiled passes (jit) hold fewer registers and take 64 KiB tiles, and
+    // their adjoint stages psi and lambda through one tile: fewer passes at
+    // about the same per-pass bandwidth (24-qubit HEA: 44 + 44 passes
+    // instead of 56 + 68)
+    int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 12 : 11) + (o.jit ? 1 : 0);
+    int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (o.jit ? (dt == 0 ? 13 : 12) : (dt == 0 ? 11 : 10));
     // fused passes need at least one full wave (64 threads) per tile
     bool fused = o.fuse != 0 && n >= R + 6;
     for (int mode = 0; mode < 2; ++mode) {

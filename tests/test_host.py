@@ -245,3 +245,29 @@ def test_brickwork_generator_matches_oracle():
     for a, b in zip(o, w):
         assert tuple(a.qubits) == tuple(b.qubits)
         np.testing.assert_array_equal(a.matrix(), b.mat)
+
+
+def test_compiled_pass_signatures(tmp_path, monkeypatch):
+    """Pass compiler on the CPU: the generated kernels declare the parameter
+    lists the launchers in jit.cpp pass (8 forward, 12 adjoint), and every
+    tile pass of a small HEA program is compiled."""
+    import re
+
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    monkeypatch.setenv("TCM_JIT_DUMP", str(tmp_path))
+    monkeypatch.setenv("TCM_CACHE_DIR", str(tmp_path / "cache"))
+    n, L = 12, 2
+    c = Circuit(n)
+    hea_layers(c, np.zeros((2 * L, n), np.float32), L)
+    specs, thetas = c._lower()
+    prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1)
+    assert prog.compile("complex64", 0) > 0.0
+    assert prog.compile("complex64", 1) > 0.0
+    for mode, name, nparams in ((0, "tcm_fwd_k", 8), (1, "tcm_adj_k", 12)):
+        srcs = "".join(p.read_text() for p in sorted(tmp_path.glob("tcm_passes_%d_*.hip" % mode)))
+        sigs = re.findall(r"\b%s\d+\(([^)]*)\)" % name, srcs)
+        assert sigs, mode
+        for s in sigs:
+            assert len(s.split(",")) == nparams, (mode, s)
