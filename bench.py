@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # copy in the same run.  Key: (qubits, layers, batch, ckpt, jit).  Commands
 # and per-counter figures: profiles/tcm_adj_passes_hea24_b32.md (jit=1) and
 # profiles/k_fused_adj_hea24_b32.md (jit=0).
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0): 17046440436, (24, 20, 32, 5, 1): 17056017082}
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0): 17046845037, (24, 20, 32, 5, 1): 17056017082}
 
 
 def parse():
@@ -369,7 +369,12 @@ def run_c2(args):
     codes, w = workloads.tfim_terms(n)
     ham = engine.PauliSum(n, codes, w)
     p = torch.as_tensor(workloads.hea_params(1002, L, n).reshape(1, -1)).cuda()
-    prog = engine.Program(n, specs, len(thetas), fuse=1)
+    t_stage = time.perf_counter()
+    prog = engine.Program(n, specs, len(thetas), fuse=1, jit=args.jit)
+    if args.jit:
+        prog.compile("complex64", 0)
+        prog.compile("complex128", 0)
+    staging = time.perf_counter() - t_stage
     st = torch.empty(1, 1 << n, dtype=torch.complex64, device="cuda")
 
     def step():
@@ -384,7 +389,8 @@ def run_c2(args):
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
            "data": "synthetic", "config": {"workload": "hea_tfim_forward", "nqubits": n, "layers": L,
-                                           "fwd_passes": prog.npasses, "seed": 1002},
+                                           "fwd_passes": prog.npasses, "seed": 1002, "jit": args.jit,
+                                           "staging_s": round(staging, 2)},
            "energy_c64": float(e32[0].real), "energy_c128": float(e64[0].real),
            "abs_denergy": abs(float(e32[0].real) - float(e64[0].real)), "fidelity_vs_c128": fid}
     print("c2: %.2f ms/eval, E=%.8f (c128 %.8f)" % (dt * 1e3, out["energy_c64"], out["energy_c128"]),
