@@ -314,7 +314,7 @@ def run_c3(args):
         c = tc.Circuit(n)
         hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
         specs, thetas = c._lower()
-        k = engine.autotune_tile(n, specs, len(thetas), batch)
+        k = min(args.tile, n) if args.tile else engine.autotune_tile(n, specs, len(thetas), batch)
         return engine.Program(n, specs, len(thetas), fuse=1, tile_bits=k), len(thetas)
 
     prog16, P16 = program(16, 8, 1024)

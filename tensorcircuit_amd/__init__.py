@@ -8,11 +8,13 @@ fallback (importing the package fails when the library is not built).
 __version__ = "0.1.0"
 
 from . import _lib  # noqa: F401  (fails loudly without libtcm.so)
-from . import cons, experimental, gates, quantum, templates
+from . import cons, experimental, gates, interfaces, quantum, templates, torchnn
 from .backend import backend
 from .circuit import Circuit, expectation
 from .cons import get_backend, runtime_dtype, set_backend, set_dtype, set_jit
 from .engine import PauliSum, Program
 from .gates import array_to_tensor, num_to_tensor
+from .interfaces import torch_interface
+from .torchnn import QuantumNet, TorchLayer
 
 K = backend

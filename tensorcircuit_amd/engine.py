@@ -459,10 +459,40 @@ class StateFn(torch.autograd.Function):
         B = info.batch_size
         N = 1 << prog.n
         P = params.shape[-1] if params.dim() else 0
-        p = _to_batch(params, in_dims[0], B, (P,)) if prog.nparams else None
+        p = _to_batch(params, in_dims[0], B, (P,))
         inp = _to_batch(inputs, in_dims[1], B, (N,)) if inputs is not None else None
-        st, ck = prog.forward(p, B, dtype, inputs=inp, want_ckpts=prog.nckpt > 0)
+        # one batched evaluation that regular autograd can also see through
+        st, ck = StateBatchFn.apply(p, inp, prog, dtype)
         return (st, ck), (0, 1)
+
+
+class StateBatchFn(torch.autograd.Function):
+    """params [B, P], inputs [B, N] or None -> (psi [B, N], checkpoints
+    [C, B, N]) in one batched launch; backward = batched reverse sweep."""
+
+    @staticmethod
+    def forward(params, inputs, prog, dtype):
+        B = params.shape[0]
+        return prog.forward(params if prog.nparams else None, B, dtype, inputs=inputs,
+                            want_ckpts=prog.nckpt > 0)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        params, inp, prog, dtype = inputs
+        st, ck = output
+        ctx.save_for_backward(params, st, ck)
+        ctx.prog, ctx.dtype, ctx.has_input = prog, dtype, inp is not None
+        ctx.mark_non_differentiable(ck)
+
+    @staticmethod
+    def backward(ctx, g_state, g_ck):
+        params, st, ck = ctx.saved_tensors
+        prog = ctx.prog
+        want_input = bool(ctx.has_input and ctx.needs_input_grad[1])
+        B = params.shape[0]
+        gp, gi = prog.vjp(params if prog.nparams else None, ck, st, g_state.contiguous(), B, ctx.dtype, want_input)
+        gp = gp.to(params.dtype).reshape(params.shape) if ctx.needs_input_grad[0] else None
+        return gp, (gi if want_input else None), None, None
 
 
 class StateVJPFn(torch.autograd.Function):
@@ -527,7 +557,27 @@ class PauliExpFn(torch.autograd.Function):
     def vmap(info, in_dims, st, ham):
         B = info.batch_size
         s = _to_batch(st, in_dims[0], B, (1 << ham.n,))
-        return ham.expect(s).to(st.dtype), 0
+        return PauliExpBatchFn.apply(s, ham), 0
+
+
+class PauliExpBatchFn(torch.autograd.Function):
+    """psi [B, N] -> <psi_b|H|psi_b> [B]; backward 2 Re(g_b) H psi_b."""
+
+    @staticmethod
+    def forward(st, ham):
+        return ham.expect(st).to(st.dtype)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, ham = inputs
+        ctx.save_for_backward(st)
+        ctx.ham = ham
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        alpha = (2.0 * torch.real(g)).to(torch.float64).reshape(-1).to(st.device)
+        return ctx.ham.apply(st, alpha), None
 
 
 class PauliExpVJPFn(torch.autograd.Function):

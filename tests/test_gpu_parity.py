@@ -603,3 +603,82 @@ def test_forward_multi_uneven(gpu):
     for n, ref in refs:
         st = [it[2] for it in items if it[0].n == n][0]
         np.testing.assert_allclose(np_of(st)[0], ref, atol=2e-6)
+
+
+# ------------------------------------------------------- torch interface ---
+
+def test_torchlayer_quantumnet(gpu):
+    """tests/test_torchnn.py:20-48: a TorchLayer over a 6-qubit classifier,
+    batch of 3 inputs -> [3, 6]; values against the oracle and weight
+    gradients against central differences of the oracle (complex128)."""
+    import tensorcircuit_amd as tc
+
+    K = tc.backend
+    n, nlayers = 6, 2
+
+    def qpred(x, weights):
+        c = tc.Circuit(n)
+        for i in range(n):
+            c.rx(i, theta=x[i])
+        for j in range(nlayers):
+            for i in range(n - 1):
+                c.cnot(i, i + 1)
+            for i in range(n):
+                c.rx(i, theta=weights[2 * j, i])
+                c.ry(i, theta=weights[2 * j + 1, i])
+        return K.real(K.stack([c.expectation_ps(x=[i]) for i in range(n)]))
+
+    torch.manual_seed(3)
+    ql = tc.TorchLayer(qpred, weights_shape=[2 * nlayers, n])
+    x = torch.rand(3, n) * 2
+    yp = ql(x)
+    assert tuple(yp.shape) == (3, n)
+
+    def oracle(xv, w):
+        ops = [sv.Op("rx", (i,), kw={"theta": float(xv[i])}) for i in range(n)]
+        for j in range(nlayers):
+            ops += [sv.Op("cnot", (i, i + 1)) for i in range(n - 1)]
+            for i in range(n):
+                ops.append(sv.Op("rx", (i,), kw={"theta": float(w[2 * j, i])}))
+                ops.append(sv.Op("ry", (i,), kw={"theta": float(w[2 * j + 1, i])}))
+        s = sv.run_ops(n, ops)
+        return np.array([np.real(sv.expectation_ps(s, n, x=[i])) for i in range(n)])
+
+    w = ql.q_weights[0].detach().cpu().numpy().astype(np.float64)
+    xs = x.numpy().astype(np.float64)
+    ref = np.stack([oracle(xs[b], w) for b in range(3)])
+    np.testing.assert_allclose(np_of(yp), ref, atol=2e-5)
+    # gradient of sum(yp) w.r.t. the weights
+    yp.sum().backward()
+    g = ql.q_weights[0].grad.detach().cpu().numpy()
+    eps = 1e-4
+    fd = np.zeros_like(w)
+    for idx in np.ndindex(*w.shape):
+        wp, wm = w.copy(), w.copy()
+        wp[idx] += eps
+        wm[idx] -= eps
+        fd[idx] = sum((oracle(xs[b], wp) - oracle(xs[b], wm)).sum() for b in range(3)) / (2 * eps)
+    np.testing.assert_allclose(g, fd, atol=2e-4)
+
+
+def test_torch_interface_backward(gpu):
+    """interfaces/torch.py docstring example: <Z> after rx(a0) ry(a1) is
+    cos(a0) cos(a1); d(<Z>^2)/da is checked against the closed form."""
+    import tensorcircuit_amd as tc
+
+    K = tc.backend
+
+    def f(params):
+        c = tc.Circuit(1)
+        c.rx(0, theta=params[0])
+        c.ry(0, theta=params[1])
+        return K.real(c.expectation((tc.gates.z(), [0])))
+
+    f_torch = tc.interfaces.torch_interface(f, jit=True)
+    a = torch.tensor([0.3, 0.7], requires_grad=True)
+    b = f_torch(a)
+    (b ** 2).backward()
+    z = np.cos(0.3) * np.cos(0.7)
+    np.testing.assert_allclose(float(b.detach()), z, atol=1e-6)
+    np.testing.assert_allclose(a.grad.numpy(), [2 * z * -np.sin(0.3) * np.cos(0.7),
+                                                2 * z * -np.cos(0.3) * np.sin(0.7)], atol=1e-5)
