@@ -161,6 +161,12 @@ int tcm_pauli_build(const int8_t* codes, const double* coeffs, int32_t nterms,
                     int32_t nqubits, tcm_pauli** out);
 int tcm_pauli_free(tcm_pauli* ham);
 
+/* Pass compiler for Pauli sums (no reference counterpart): compile the
+ * tiled passes of `ham` for dtype into straight-line kernels, used by every
+ * later expectation / apply / adjoint call with this sum; needs no device.
+ * *seconds (nullable): compile time, 0 when cached. */
+int tcm_pauli_compile(tcm_pauli* ham, int32_t dtype, double* seconds);
+
 /* Scratch bytes for expectation / apply over `batch` states. */
 int tcm_pauli_workspace_bytes(const tcm_pauli* ham, int32_t dtype,
                               int64_t batch, int64_t* bytes);

@@ -91,6 +91,7 @@ SIGNATURES = {
     "tcm_forward": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _i32, _vp, _vp, _i64, _vp]),
     "tcm_pauli_build": (C.c_int, [C.POINTER(C.c_int8), _dp, _i32, _i32, C.POINTER(_vp)]),
     "tcm_pauli_free": (C.c_int, [_vp]),
+    "tcm_pauli_compile": (C.c_int, [_vp, _i32, C.POINTER(C.c_double)]),
     "tcm_pauli_workspace_bytes": (C.c_int, [_vp, _i32, _i64, C.POINTER(_i64)]),
     "tcm_expect_pauli_sum": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tcm_apply_pauli_sum": (C.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp]),

@@ -24,6 +24,16 @@ struct PauliDev {
   void* tgroups[2] = {nullptr, nullptr};
   void* tterms[2] = {nullptr, nullptr};
   void* truns[2] = {nullptr, nullptr};
+  // compiled tile passes (per dtype): modules and one kernel per pass
+  std::vector<void*> jmods[2], jfns[2];
+};
+
+// compiled tile passes of a Pauli sum for one dtype
+struct PauliJit {
+  int state = 0;  // 0 none, 1 built, -1 failed
+  std::vector<std::vector<char>> code;
+  std::vector<int32_t> chunk;
+  double seconds = 0.0;
 };
 
 // LDS-tiled schedule of a Pauli sum for one dtype
@@ -44,12 +54,19 @@ struct tcm_pauli {
   std::vector<DPTerm> terms;
   int nslots = 0;                  // slots of the per-group kernels
   PauliTileSched ts[2];
+  PauliJit jit[2];
   std::mutex mu;
   std::vector<PauliDev> dev;
 };
 
 namespace tcm {
 std::string describe_sched(const Program& P, int dtype, int mode);
+std::string gen_pauli_source(int n, int dt, const std::vector<DPPass>& passes, const std::vector<DPGroupT>& groups,
+                             const std::vector<DPTermT>& terms, std::vector<std::string>& bodies);
+int compile_bodies(const std::string& prelude, const std::vector<std::string>& bodies, const std::string& dump_tag,
+                   std::vector<std::vector<char>>& code, std::vector<int32_t>& chunk_of, double* seconds);
+int load_modules(const std::vector<std::vector<char>>& code, const std::vector<int32_t>& chunk_of,
+                 const std::string& prefix, std::vector<void*>& mods_out, std::vector<void*>& fns_out);
 
 static thread_local std::string g_err;
 
@@ -252,7 +269,32 @@ int ensure_pauli(tcm_pauli* H, int dt, PauliDev** out) {
     rc = rc ? rc : upload(S.runoff, &pd.truns[dt]);
     if (rc) return rc;
   }
+  if (H->jit[dt].state == 1 && pd.jfns[dt].empty() && !H->jit[dt].chunk.empty())
+    if (int rc = load_modules(H->jit[dt].code, H->jit[dt].chunk, "tcm_pauli_k", pd.jmods[dt], pd.jfns[dt])) return rc;
   *out = &pd;
+  return 0;
+}
+
+// compile the tiled passes of H for dtype dt (caller holds no lock)
+int pauli_jit_build(tcm_pauli* H, int dt, double* seconds) {
+  std::lock_guard<std::mutex> lk(H->mu);
+  PauliJit& J = H->jit[dt];
+  if (seconds) *seconds = 0.0;
+  if (J.state == 1) return 0;
+  if (J.state < 0) return set_error(TCM_E_UNSUPPORTED, "pass compiler failed for this Pauli sum");
+  const PauliTileSched& S = H->ts[dt];
+  if (!S.ok) {
+    J.state = 1;  // nothing tiled: the per-group kernels run
+    return 0;
+  }
+  std::vector<std::string> bodies;
+  const std::string prelude = gen_pauli_source(H->n, dt, S.passes, S.groups, S.terms, bodies);
+  if (int rc = compile_bodies(prelude, bodies, "pauli", J.code, J.chunk, &J.seconds)) {
+    J.state = -1;
+    return rc;
+  }
+  J.state = 1;
+  if (seconds) *seconds = J.seconds;
   return 0;
 }
 
@@ -283,10 +325,27 @@ int pauli_energy_apply(tcm_pauli* H, int dt, const void* psi, void* out, const d
     return 0;
   }
   bool first = true;
+  const bool jit = !pd->jfns[dt].empty();
   for (size_t pi = 0; pi < S.passes.size(); ++pi) {
-    launch_pauli_tile<T>(S.passes[pi], (const DPPass*)pd->tpasses[dt], (int)pi, (const DPGroupT*)pd->tgroups[dt],
-                         (const DPTermT*)pd->tterms[dt], (const uint32_t*)pd->truns[dt], psi, first ? nullptr : out,
-                         out, H->n, alpha, epart, estride, batch, energy, apply, st);
+    if (jit) {
+      // parameter list of tcm_pauli_k<pi> (passgen.cpp)
+      const void* a_psi = psi;
+      const void* a_lin = first ? nullptr : out;
+      void* a_lout = out;
+      long long sstride = 1ll << H->n, es = estride;
+      const double* a_al = alpha;
+      double* a_ep = epart;
+      long long total = batch << S.passes[pi].nblk_log;
+      int flags = (energy ? 1 : 0) | (apply ? 2 : 0);
+      void* args[] = {&a_psi, &a_lin, &a_lout, &sstride, &a_al, &a_ep, &es, &total, &flags};
+      const hipError_t e = hipModuleLaunchKernel((hipFunction_t)pd->jfns[dt][pi], (unsigned)total, 1, 1,
+                                                 1u << S.passes[pi].nthr_log, 1, 1, 0, st, args, nullptr);
+      if (e != hipSuccess) return set_error((int)e, "hipModuleLaunchKernel(pauli): %s", hipGetErrorString(e));
+    } else {
+      launch_pauli_tile<T>(S.passes[pi], (const DPPass*)pd->tpasses[dt], (int)pi, (const DPGroupT*)pd->tgroups[dt],
+                           (const DPTermT*)pd->tterms[dt], (const uint32_t*)pd->truns[dt], psi,
+                           first ? nullptr : out, out, H->n, alpha, epart, estride, batch, energy, apply, st);
+    }
     first = false;
   }
   for (size_t i = 0; i < S.legacy.size(); ++i) {
@@ -508,6 +567,9 @@ int energy_grad_impl(Program& P, int dtype, const void* params, int64_t pstride,
   DeviceTables* DA = nullptr;
   if (int rc = ensure_tables(P, dtype, 0, &DF)) return rc;
   if (int rc = ensure_tables(P, dtype, 1, &DA)) return rc;
+  // a program built with the pass compiler compiles the Hamiltonian too
+  if (P.opts.jit)
+    if (int rc = pauli_jit_build(H, dtype, nullptr)) return rc;
   const Sched& F = P.sched[dtype][0];
   const Sched& A = P.sched[dtype][1];
   const int64_t N = 1ll << P.n;
@@ -798,12 +860,19 @@ int tcm_pauli_free(tcm_pauli* ham) {
       void* ps[] = {pd.tpasses[dt], pd.tgroups[dt], pd.tterms[dt], pd.truns[dt]};
       for (void* q : ps)
         if (q) (void)hipFree(q);
+      for (void* m : pd.jmods[dt])
+        if (m) (void)hipModuleUnload((hipModule_t)m);
     }
   }
   if (have) (void)hipSetDevice(cur);
   (void)hipGetLastError();
   delete ham;
   return 0;
+}
+
+int tcm_pauli_compile(tcm_pauli* ham, int32_t dtype, double* seconds) {
+  if (!ham || !valid_dtype(dtype)) return set_error(TCM_E_ARG, "bad compile args");
+  return pauli_jit_build(ham, dtype, seconds);
 }
 
 int tcm_pauli_workspace_bytes(const tcm_pauli* ham, int32_t dtype, int64_t batch, int64_t* bytes) {

@@ -113,35 +113,36 @@ void rtc_compile(const std::string& src, const std::vector<std::string>& opts, s
 
 }  // namespace
 
-int jit_build(const Program& P, Sched& S, int dt, int mode) {
+// Compile kernel bodies (each a complete __global__ definition) behind a
+// shared prelude into code objects: bodies are dealt round-robin into up
+// to 16 chunks compiled on parallel threads; each chunk is cached on disk
+// under a hash of its text and the compiler options.  On success code[c]
+// holds chunk c and chunk_of[i] the chunk of body i.
+int compile_bodies(const std::string& prelude, const std::vector<std::string>& bodies, const std::string& dump_tag,
+                   std::vector<std::vector<char>>& code, std::vector<int32_t>& chunk_of, double* seconds) {
   const auto t0 = std::chrono::steady_clock::now();
-  S.jit_prefix = mode == 0 ? "tcm_fwd_k" : "tcm_adj_k";
-  std::vector<std::string> bodies;
-  const std::string prelude = gen_pass_source(P, S, dt, mode, S.jit, bodies);
-  if (bodies.empty()) {
-    S.jit_state = 1;
-    return 0;
-  }
+  *seconds = 0.0;
+  code.clear();
+  chunk_of.assign(bodies.size(), 0);
+  if (bodies.empty()) return 0;
   const std::string arch = device_arch();
   const std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17", "-ffp-contract=fast",
                                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
   std::string okey = arch;
   for (const auto& o : opts) okey += " " + o;
-  // chunks compiled in parallel, each a code object of its own
   unsigned hw = std::thread::hardware_concurrency();
   const int nch = (int)std::max<size_t>(1, std::min<size_t>({bodies.size(), (size_t)(hw ? hw : 4), (size_t)16}));
   std::vector<std::string> srcs(nch, prelude);
-  S.jit_chunk.assign(bodies.size(), 0);
   for (size_t i = 0; i < bodies.size(); ++i) {
-    S.jit_chunk[i] = (int)(i % nch);
+    chunk_of[i] = (int)(i % nch);
     srcs[i % nch] += bodies[i];
   }
   if (const char* dump = getenv("TCM_JIT_DUMP"))
     for (int c = 0; c < nch; ++c) {
-      std::ofstream f(std::string(dump) + "/tcm_passes_" + std::to_string(mode) + "_" + std::to_string(c) + ".hip");
+      std::ofstream f(std::string(dump) + "/tcm_passes_" + dump_tag + "_" + std::to_string(c) + ".hip");
       f << srcs[c];
     }
-  S.jit_code.assign(nch, std::vector<char>());
+  code.assign(nch, std::vector<char>());
   std::vector<std::string> errs(nch);
   const std::string dir = cache_dir();
   std::vector<std::string> paths(nch);
@@ -151,45 +152,58 @@ int jit_build(const Program& P, Sched& S, int dt, int mode) {
       char h[32];
       snprintf(h, sizeof h, "%016llx", (unsigned long long)fnv1a(srcs[c], fnv1a(okey)));
       paths[c] = dir + "/" + h + ".co";
-      if (read_file(paths[c], S.jit_code[c])) continue;
+      if (read_file(paths[c], code[c])) continue;
     }
     todo.push_back(c);
   }
   std::vector<std::thread> th;
-  for (int c : todo) th.emplace_back(rtc_compile, std::cref(srcs[c]), std::cref(opts), &S.jit_code[c], &errs[c]);
+  for (int c : todo) th.emplace_back(rtc_compile, std::cref(srcs[c]), std::cref(opts), &code[c], &errs[c]);
   for (auto& t : th) t.join();
   for (int c : todo)
-    if (!errs[c].empty()) {
-      S.jit_state = -1;
-      return set_error(TCM_E_UNSUPPORTED, "pass compiler: %s", errs[c].c_str());
-    }
+    if (!errs[c].empty()) return set_error(TCM_E_UNSUPPORTED, "pass compiler: %s", errs[c].c_str());
   if (!dir.empty() && !todo.empty()) {
     mkdirs(dir);
-    for (int c : todo) write_file(paths[c], S.jit_code[c]);
+    for (int c : todo) write_file(paths[c], code[c]);
   }
-  S.jit_seconds = todo.empty() ? 0.0
-                               : std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *seconds = todo.empty() ? 0.0 : std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int jit_build(const Program& P, Sched& S, int dt, int mode) {
+  S.jit_prefix = mode == 0 ? "tcm_fwd_k" : "tcm_adj_k";
+  std::vector<std::string> bodies;
+  const std::string prelude = gen_pass_source(P, S, dt, mode, S.jit, bodies);
+  if (int rc = compile_bodies(prelude, bodies, std::to_string(mode), S.jit_code, S.jit_chunk, &S.jit_seconds)) {
+    S.jit_state = -1;
+    return rc;
+  }
   S.jit_state = 1;
+  return 0;
+}
+
+// load code objects and look up kernels prefix + index (index i in chunk_of[i])
+int load_modules(const std::vector<std::vector<char>>& code, const std::vector<int32_t>& chunk_of,
+                 const std::string& prefix, std::vector<void*>& mods_out, std::vector<void*>& fns_out) {
+  std::vector<hipModule_t> mods(code.size(), nullptr);
+  for (size_t c = 0; c < code.size(); ++c) {
+    const hipError_t e = hipModuleLoadData(&mods[c], code[c].data());
+    if (e != hipSuccess) return set_error((int)e, "hipModuleLoadData: %s", hipGetErrorString(e));
+    mods_out.push_back((void*)mods[c]);
+  }
+  fns_out.assign(chunk_of.size(), nullptr);
+  for (size_t k = 0; k < chunk_of.size(); ++k) {
+    hipFunction_t f;
+    const std::string name = prefix + std::to_string(k);
+    const hipError_t e = hipModuleGetFunction(&f, mods[chunk_of[k]], name.c_str());
+    if (e != hipSuccess) return set_error((int)e, "hipModuleGetFunction(%s): %s", name.c_str(), hipGetErrorString(e));
+    fns_out[k] = (void*)f;
+  }
   return 0;
 }
 
 int jit_load(const Sched& S, DeviceTables& D) {
   if (S.jit_state != 1 || !D.jit_fn.empty()) return 0;
-  std::vector<hipModule_t> mods(S.jit_code.size(), nullptr);
-  for (size_t c = 0; c < S.jit_code.size(); ++c) {
-    const hipError_t e = hipModuleLoadData(&mods[c], S.jit_code[c].data());
-    if (e != hipSuccess) return set_error((int)e, "hipModuleLoadData: %s", hipGetErrorString(e));
-    D.jit_mod.push_back((void*)mods[c]);
-  }
-  D.jit_fn.assign(S.jit_chunk.size(), nullptr);
-  for (size_t k = 0; k < S.jit_chunk.size(); ++k) {
-    hipFunction_t f;
-    const std::string name = S.jit_prefix + std::to_string(k);
-    const hipError_t e = hipModuleGetFunction(&f, mods[S.jit_chunk[k]], name.c_str());
-    if (e != hipSuccess) return set_error((int)e, "hipModuleGetFunction(%s): %s", name.c_str(), hipGetErrorString(e));
-    D.jit_fn[k] = (void*)f;
-  }
-  return 0;
+  return load_modules(S.jit_code, S.jit_chunk, S.jit_prefix, D.jit_mod, D.jit_fn);
 }
 
 int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const void* mtab, const void* ctab,

@@ -19,19 +19,23 @@
 #include <unordered_map>
 #include <vector>
 
-#include "tcm_internal.h"
+#include "kernels.h"
 
 namespace tcm {
 
 namespace {
 
 std::string fmt(const char* f, ...) {
-  char buf[1024];
   va_list ap;
   va_start(ap, f);
-  vsnprintf(buf, sizeof buf, f, ap);
+  va_list ap2;
+  va_copy(ap2, ap);
+  const int len = vsnprintf(nullptr, 0, f, ap);
   va_end(ap);
-  return buf;
+  std::string out(len > 0 ? (size_t)len : 0, '\0');
+  if (len > 0) vsnprintf(&out[0], (size_t)len + 1, f, ap2);
+  va_end(ap2);
+  return out;
 }
 
 uint32_t swz_h(uint32_t t) { return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu); }
@@ -646,6 +650,149 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
   }
   src = std::string("#define REAL ") + (dt == 0 ? "float" : "double") + "\n" + kPrelude;
   return src;
+}
+
+}  // namespace tcm
+
+namespace tcm {
+
+// ---------------------------------------------------------------------------
+// Pauli-sum tile passes, compiled: for every pass of the tiled Pauli schedule
+// a kernel that reads psi once into LDS and evaluates h = sum of the pass's
+// terms (P psi)[y] = i^ny (-1)^popc((y ^ flip) & yz) psi[y ^ flip] with all
+// masks compile-time.  Per term, the sign splits into a block-uniform part
+// (bits outside the tile), a per-thread part (thread bits) and a per-element
+// constant, so a thread sums each group's weights once per sign pattern and
+// spends a few adds per element.  Semantics are those of k_pauli_tile.
+// ---------------------------------------------------------------------------
+std::string gen_pauli_source(int n, int dt, const std::vector<DPPass>& passes, const std::vector<DPGroupT>& groups,
+                             const std::vector<DPTermT>& terms, std::vector<std::string>& bodies) {
+  bodies.clear();
+  const uint32_t cb = dt == 0 ? 8u : 16u;
+  for (size_t pi = 0; pi < passes.size(); ++pi) {
+    const DPPass& P = passes[pi];
+    const int k = P.k, tl = P.nthr_log, NT = 1 << tl, nbl = P.nblk_log, NW = NT / 64;
+    const int nit = 1 << (k - tl);
+    // tile bits: the bits not in blk_bits, ascending
+    std::vector<int> tile;
+    {
+      std::vector<char> isb(n, 0);
+      for (int i = 0; i < nbl; ++i) isb[P.blk_bits[i]] = 1;
+      for (int b = 0; b < n; ++b)
+        if (!isb[b]) tile.push_back(b);
+    }
+    std::vector<int> bs, bd, ts, td;
+    for (int i = 0; i < nbl; ++i) {
+      bs.push_back(i);
+      bd.push_back(P.blk_bits[i]);
+    }
+    for (int t = 0; t < tl; ++t) {
+      ts.push_back(t);
+      td.push_back(tile[t]);
+    }
+    auto hi = [&](int it) {
+      unsigned long long o = 0;
+      for (int t = tl; t < k; ++t)
+        if (((uint64_t)it << tl) >> t & 1) o |= 1ull << tile[t];
+      return o;
+    };
+    std::string o;
+    auto L = [&](const std::string& s) { o += s; o += '\n'; };
+    L(fmt("extern \"C\" __global__ void __launch_bounds__(%d) tcm_pauli_k%d(const cT* __restrict__ psi, "
+          "const cT* lam_in, cT* lam_out, long long sstride, const double* __restrict__ alpha, "
+          "double* __restrict__ epart, long long estride, long long total, int flags) {",
+          NT, (int)pi));
+    L("if ((unsigned long long)blockIdx.x >= (unsigned long long)total) return;");
+    L(fmt("__shared__ cT tile[%d];", 1 << k));
+    L(fmt("__shared__ double red[%d];", 2 * NW));
+    L("const unsigned tid = threadIdx.x;");
+    L(fmt("const unsigned long long blk = blockIdx.x & 0x%llxull;", nbl >= 64 ? ~0ull : ((1ull << nbl) - 1)));
+    L(fmt("const unsigned long long b = (unsigned long long)blockIdx.x >> %d;", nbl));
+    L(fmt("const unsigned long long base = %s;", deposit_expr("blk", bs, bd, "unsigned long long").c_str()));
+    L(fmt("const unsigned long long dlo = %s;", deposit_expr("tid", ts, td, "unsigned long long").c_str()));
+    L("const unsigned slo = swz(tid) * (unsigned)sizeof(cT);");
+    L("const cT* __restrict__ s = psi + b * sstride + base;");
+    for (int it = 0; it < nit; ++it) L(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
+    for (int it = 0; it < nit; ++it) L(fmt("LDSB(tile, slo ^ %uu) = r%d;", swz_h((uint32_t)it << tl) * cb, it));
+    L("__syncthreads();");
+    for (int it = 0; it < nit; ++it) L(fmt("T hr%d = 0, hi%d = 0;", it, it));
+    const uint32_t tmask = (uint32_t)NT - 1;
+    for (int g = P.g0; g < P.g0 + P.ng; ++g) {
+      const DPGroupT& G = groups[g];
+      const uint32_t flo = G.flip_t & tmask, fhi = G.flip_t >> tl;
+      L("{");
+      // per-thread weight sums by element sign pattern: pattern -> (re, im)
+      std::vector<uint32_t> pats;
+      std::vector<std::string> sre, sim;
+      for (int t = G.t0; t < G.t0 + G.nt; ++t) {
+        const DPTermT& tm = terms[t];
+        const uint32_t ylo = tm.yz_t & tmask, pat = tm.yz_t >> tl;
+        size_t pidx = 0;
+        while (pidx < pats.size() && pats[pidx] != pat) ++pidx;
+        if (pidx == pats.size()) {
+          pats.push_back(pat);
+          sre.push_back("");
+          sim.push_back("");
+        }
+        // sign bit: popc(base & yz_g) + popc((tid ^ flip_lo) & yz_lo)
+        std::string sb = fmt("((__builtin_popcountll(base & 0x%llxull) + __builtin_popcount((tid ^ %uu) & %uu)) & 1)",
+                             (unsigned long long)tm.yz_g, flo, ylo);
+        const double w = tm.w;
+        // i^ny * w: 0 -> +re, 1 -> +im, 2 -> -re, 3 -> -im
+        const int q = tm.ny & 3;
+        const double ww = (q >= 2) ? -w : w;
+        const std::string term = fmt("(%s ? (T)(%.17g) : (T)(%.17g))", sb.c_str(), -ww, ww);
+        if (q == 0 || q == 2) sre[pidx] += (sre[pidx].empty() ? "" : " + ") + term;
+        else sim[pidx] += (sim[pidx].empty() ? "" : " + ") + term;
+      }
+      for (size_t pidx = 0; pidx < pats.size(); ++pidx)
+        L(fmt("const T sr%d = %s, si%d = %s;", (int)pidx, sre[pidx].empty() ? "(T)0" : sre[pidx].c_str(), (int)pidx,
+              sim[pidx].empty() ? "(T)0" : sim[pidx].c_str()));
+      if (G.flip_t) L(fmt("const unsigned pa = swz(tid ^ %uu) * (unsigned)sizeof(cT);", flo));
+      for (int it = 0; it < nit; ++it) {
+        std::string cr, ci;
+        for (size_t pidx = 0; pidx < pats.size(); ++pidx) {
+          const bool neg = __builtin_popcount(((uint32_t)it ^ fhi) & pats[pidx]) & 1;
+          cr += fmt(" %s sr%d", neg ? "-" : "+", (int)pidx);
+          ci += fmt(" %s si%d", neg ? "-" : "+", (int)pidx);
+        }
+        std::string qv = G.flip_t ? fmt("LDSB(tile, pa ^ %uu)", swz_h(((uint32_t)it ^ fhi) << tl) * cb)
+                                  : fmt("r%d", it);
+        L(fmt("{ const cT q = %s; const T cr = (T)0%s, ci = (T)0%s;", qv.c_str(), cr.c_str(), ci.c_str()));
+        L(fmt("  hr%d += cr * q.x - ci * q.y; hi%d += cr * q.y + ci * q.x; }", it, it));
+      }
+      L("}");
+    }
+    // energy partials and lambda update
+    L("if (flags & 1) {");
+    L("  double er = 0.0, ei = 0.0;");
+    for (int it = 0; it < nit; ++it) {
+      L(fmt("  er += (double)r%d.x * hr%d + (double)r%d.y * hi%d;", it, it, it, it));
+      L(fmt("  ei += (double)r%d.x * hi%d - (double)r%d.y * hr%d;", it, it, it, it));
+    }
+    L("  er = wsum(er); ei = wsum(ei);");
+    L("  if ((tid & 63u) == 0) { red[2 * (tid >> 6)] = er; red[2 * (tid >> 6) + 1] = ei; }");
+    L("}");
+    L("__syncthreads();");
+    L("if ((flags & 1) && tid == 0) {");
+    L("  double a = 0.0, c = 0.0;");
+    L(fmt("  for (int w = 0; w < %d; ++w) { a += red[2 * w]; c += red[2 * w + 1]; }", NW));
+    L(fmt("  epart[b * estride + 2 * (%d + blk)] = a;", P.slot0));
+    L(fmt("  epart[b * estride + 2 * (%d + blk) + 1] = c;", P.slot0));
+    L("}");
+    L("if (flags & 2) {");
+    L("  const double al = alpha ? alpha[b] : 1.0;");
+    L("  const unsigned long long ob = b * sstride + base + dlo;");
+    for (int it = 0; it < nit; ++it) {
+      L(fmt("  { const unsigned long long off = ob + %lluull;", hi(it)));
+      L("    const cT o = lam_in ? lam_in[off] : cT{(T)0, (T)0};");
+      L(fmt("    lam_out[off] = cT{(T)(o.x + al * hr%d), (T)(o.y + al * hi%d)}; }", it, it));
+    }
+    L("}");
+    L("}");
+    bodies.push_back(o);
+  }
+  return std::string("#define REAL ") + (dt == 0 ? "float" : "double") + "\n" + kPrelude;
 }
 
 }  // namespace tcm

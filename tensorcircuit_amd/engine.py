@@ -333,7 +333,9 @@ class PauliSum:
     """Sum_k w_k P_k with P_k given by codes 0=I,1=X,2=Y,3=Z per qubit
     (the ``ps`` structures of quantum.py:1025-1068 / any_measurements)."""
 
-    def __init__(self, n: int, codes: Any, coeffs: Any):
+    def __init__(self, n: int, codes: Any, coeffs: Any, jit: Optional[int] = None):
+        """``jit``: compile the tiled passes (1), keep the generic kernel (0),
+        or follow the package policy for the qubit count (None)."""
         codes = np.ascontiguousarray(np.asarray(codes, dtype=np.int8).reshape(-1, n))
         coeffs = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64).reshape(-1))
         if codes.shape[0] != coeffs.shape[0]:
@@ -346,6 +348,24 @@ class PauliSum:
                                   coeffs.ctypes.data_as(C.POINTER(C.c_double)), codes.shape[0], n, C.byref(h)),
               "tcm_pauli_build")
         self.handle = h
+        if jit is None:
+            from . import cons
+
+            jit = cons.jit_for(n)
+        self.jit = int(jit)
+        self._compiled = set()
+
+    def compile(self, dtype: str = "complex64") -> float:
+        """Compile the tiled passes for dtype now; returns compile seconds."""
+        sec = C.c_double(0.0)
+        check(lib.tcm_pauli_compile(self.handle, _CODE[dtype], C.byref(sec)), "tcm_pauli_compile")
+        self._compiled.add(_CODE[dtype])
+        return sec.value
+
+    def _prepare(self, code: int) -> None:
+        if self.jit and code not in self._compiled:
+            check(lib.tcm_pauli_compile(self.handle, code, None), "tcm_pauli_compile")
+            self._compiled.add(code)
 
     def __del__(self, _free=lib.tcm_pauli_free):
         h = getattr(self, "handle", None)
@@ -357,6 +377,7 @@ class PauliSum:
         """complex128 [batch] of <psi|H|psi> (fp64 accumulation)."""
         dev = device()
         code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
+        self._prepare(code)
         s = state.reshape(-1, 1 << self.n).contiguous()
         b = s.shape[0]
         out = torch.empty(b, 2, dtype=torch.float64, device=dev)
@@ -370,6 +391,7 @@ class PauliSum:
     def apply(self, state: torch.Tensor, alpha: Optional[torch.Tensor] = None) -> torch.Tensor:
         """alpha_b * H psi_b as a new state tensor."""
         code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
+        self._prepare(code)
         s = state.reshape(-1, 1 << self.n).contiguous()
         out = torch.empty_like(s)
         a = None if alpha is None else alpha.to(torch.float64).contiguous()

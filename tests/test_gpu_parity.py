@@ -456,7 +456,9 @@ def test_parameter_shift_grad_matches_adjoint(gpu, highp):
 
 # ----------------------------------------------------------------- Pauli ---
 
-def test_pauli_sums_vs_oracle(gpu):
+@pytest.mark.parametrize("jit", [0, 1])
+def test_pauli_sums_vs_oracle(gpu, jit):
+    """Generic (jit=0) and compiled (jit=1) tiled Pauli passes."""
     from tensorcircuit_amd import engine
 
     rng = np.random.default_rng(99)
@@ -465,7 +467,7 @@ def test_pauli_sums_vs_oracle(gpu):
         s /= np.linalg.norm(s)
         codes = rng.integers(0, 4, size=(T, n)).astype(np.int8)
         w = rng.standard_normal(T)
-        ham = engine.PauliSum(n, codes, w)
+        ham = engine.PauliSum(n, codes, w, jit=jit)
         terms = [(float(w[k]), list(codes[k])) for k in range(T)]
         for dt in (torch.complex64, torch.complex128):
             st = torch.as_tensor(s).to(dt).cuda()
@@ -478,7 +480,8 @@ def test_pauli_sums_vs_oracle(gpu):
             np.testing.assert_allclose(hs, sv.pauli_sum_apply(s, n, terms), atol=tol * 10 * np.abs(w).sum())
 
 
-def test_pauli_wide_terms_and_tiles(gpu):
+@pytest.mark.parametrize("jit", [0, 1])
+def test_pauli_wide_terms_and_tiles(gpu, jit):
     """Terms wider than an LDS tile (X on every qubit) run beside tiled ones."""
     from tensorcircuit_amd import engine
 
@@ -490,7 +493,7 @@ def test_pauli_wide_terms_and_tiles(gpu):
     codes += [list(rng.integers(0, 4, n)) for _ in range(6)]
     w = rng.standard_normal(len(codes))
     terms = [(float(w[k]), list(codes[k])) for k in range(len(codes))]
-    ham = engine.PauliSum(n, codes, w)
+    ham = engine.PauliSum(n, codes, w, jit=jit)
     for dt, tol in ((torch.complex64, 1e-5), (torch.complex128, 1e-12)):
         st = torch.as_tensor(s).to(dt).cuda()
         e = np_of(ham.expect(st))[0]
@@ -499,6 +502,28 @@ def test_pauli_wide_terms_and_tiles(gpu):
         alpha = torch.tensor([0.5], dtype=torch.float64, device="cuda")
         hs = np_of(ham.apply(st, alpha))[0]
         np.testing.assert_allclose(hs, 0.5 * sv.pauli_sum_apply(s, n, terms), atol=tol * 10 * np.abs(w).sum())
+
+
+def test_pauli_compiled_batch(gpu):
+    """Compiled Pauli passes on a batch of states: energies and alpha-scaled
+    H psi per item against the oracle (complex128, 16 qubits, Y terms)."""
+    from tensorcircuit_amd import engine
+
+    rng = np.random.default_rng(12)
+    n, B, T = 16, 3, 12
+    codes = rng.integers(0, 4, size=(T, n)).astype(np.int8)
+    w = rng.standard_normal(T)
+    terms = [(float(w[k]), list(codes[k])) for k in range(T)]
+    ham = engine.PauliSum(n, codes, w, jit=1)
+    s = rng.standard_normal((B, 2**n)) + 1j * rng.standard_normal((B, 2**n))
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    st = torch.as_tensor(s).cuda()
+    e = np_of(ham.expect(st))
+    alpha = torch.tensor([0.5, -1.0, 2.0], dtype=torch.float64, device="cuda")
+    hs = np_of(ham.apply(st, alpha))
+    for b in range(B):
+        assert abs(e[b] - sv.pauli_sum_expectation(s[b], n, terms)) < 1e-12 * np.abs(w).sum()
+        np.testing.assert_allclose(hs[b], float(alpha[b]) * sv.pauli_sum_apply(s[b], n, terms), atol=1e-11)
 
 
 def test_heisenberg_ring_energy(gpu):
