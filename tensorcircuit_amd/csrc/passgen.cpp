@@ -85,6 +85,34 @@ __device__ __forceinline__ double read_lane(double x, int lane) {
   r.i[1] = __builtin_amdgcn_readlane(u.i[1], lane);
   return r.d;
 }
+// fp64 sum over each 16-lane row: the row total lands in its last lane
+__device__ __forceinline__ double rowsum(double x) {
+  x += dpp_shr<0x111>(x);
+  x += dpp_shr<0x112>(x);
+  x += dpp_shr<0x114>(x);
+  x += dpp_shr<0x118>(x);
+  return x;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows(double x) {
+  tcm_d2i u, r;
+  u.d = x;
+  r.i[0] = __builtin_amdgcn_update_dpp(0, u.i[0], CTRL, ROWS, 0xF, false);
+  r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], CTRL, ROWS, 0xF, false);
+  return r.d;
+}
+// fp64 sum over the wave, total in lane 63 only: row prefix sums by
+// row_shr 1/2/4/8, then row_bcast:15 into rows 1 and 3 and row_bcast:31
+// into rows 2 and 3 (other rows add the zero `old` value)
+__device__ __forceinline__ double wsum63(double x) {
+  x += dpp_shr<0x111>(x);
+  x += dpp_shr<0x112>(x);
+  x += dpp_shr<0x114>(x);
+  x += dpp_shr<0x118>(x);
+  x += dpp_rows<0x142, 0xA>(x);
+  x += dpp_rows<0x143, 0xC>(x);
+  return x;
+}
 // fp64 sum over a 64-lane wave (uniform result)
 __device__ __forceinline__ double wsum(double x) {
   x += dpp_shr<0x111>(x);
@@ -368,6 +396,17 @@ struct Gen {
   }
 };
 
+// Gradient partials per trainable gate and block in an adjoint pass: one
+// per 16-lane row (fewest instructions) when that LDS fits as many blocks
+// per CU as one per wave does, else one per wave.
+int adj_partials(int dt, const DPass& dp) {
+  const int NT = 1 << dp.tthreads_log;
+  const size_t tile = (size_t)(dt == 0 ? 8 : 16) << dp.k;
+  const size_t g = (size_t)std::max(1, dp.ngop);
+  auto blocks = [&](int per) { return (160u * 1024u) / (tile + 8 * g * (size_t)per); };
+  return blocks(NT / 16) == blocks(NT / 64) ? NT / 16 : NT / 64;
+}
+
 bool pass_supported(const Sched& S, const DPass& dp) {
   if (dp.plain_op >= 0 || dp.k <= 0) return false;
   if ((1 << dp.tthreads_log) < 64 || dp.tthreads_log > 10) return false;
@@ -392,6 +431,13 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   const int k = dp.k, tl = dp.tthreads_log, NT = 1 << tl, nbl = dp.nblk_log;
   const bool wide = P.n > 31;
   const uint32_t cb = dt == 0 ? 8u : 16u;  // bytes per complex amplitude
+  // gradient partials: per 16-lane row (rowsum, last lane of each row) or
+  // per wave (wsum63, lane 63)
+  const int PR = adj ? adj_partials(dt, dp) : 1;
+  const bool rows = PR == NT / 16;
+  const char* rsum = rows ? "rowsum" : "wsum63";
+  const uint32_t lmask = rows ? 15u : 63u;
+  const int lshift = rows ? 4 : 6;
   const char* IT = wide ? "unsigned long long" : "unsigned";
   int mb = -1;
   for (int g = 0; g < dp.ngroups; ++g) {
@@ -452,15 +498,15 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
               if (sg) e += fmt(" %s im%d", sg > 0 ? "+" : "-", j);
             }
             if (e.empty()) e = " + 0";
-            G.line(fmt("{ const T ga = (T)0%s; const double gs = wsum((double)ga);", e.c_str()));
-            G.line(fmt("  if (lane == 0) red[%d + wv] = gs; }", M.gloc * (NT / 64)));
+            G.line(fmt("{ const T ga = (T)0%s; const double gs = %s((double)ga);", e.c_str(), rsum));
+            G.line(fmt("  if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs; }", lmask, lmask, M.gloc * PR, lshift));
           }
           G.line("}");
         } else if (op.gen != GN_NONE) {
           G.line("{");
           if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return false;
-          G.line("const double gs = wsum((double)ga);");
-          G.line(fmt("if (lane == 0) red[%d + wv] = gs;", op.gloc * (NT / 64)));
+          G.line(fmt("const double gs = %s((double)ga);", rsum));
+          G.line(fmt("if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs;", lmask, lmask, op.gloc * PR, lshift));
           G.line("}");
         }
         G.apply(op, {"p", "l"}, opid++);
@@ -472,8 +518,8 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     // through it, then lambda (four barriers instead of one), for half the
     // LDS of two tiles and twice the resident blocks per CU.
     G.line(fmt("__shared__ cT tb[%d];", 1 << k));
-    G.line(fmt("__shared__ double red[%d];", std::max(1, dp.ngop) * (NT / 64)));
-    G.line("const unsigned lane = tid & 63u, wv = tid >> 6;");
+    // gradient partials: one fp64 per wave and trainable gate
+    G.line(fmt("__shared__ double red[%d];", std::max(1, dp.ngop) * PR));
     {
       std::string decl = "cT";
       for (int j = 0; j < nreg; ++j) decl += fmt("%s p%d, l%d", j ? "," : "", j, j);
@@ -555,7 +601,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     if (dp.ngop > 0) {
       G.line(fmt("for (unsigned q = tid; q < %du; q += %du) {", dp.ngop, NT));
       G.line("  double sum = 0.0;");
-      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += red[q * %d + w];", NT / 64, NT / 64));
+      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += red[q * %d + w];", PR, PR));
       G.line(fmt("  part[b * pstride + (long long)q * %lldll + (long long)blk] = sum;", 1ll << nbl));
       G.line("}");
     }
@@ -624,7 +670,7 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
       // (160 KiB per CU), so registers never cut occupancy below that
       const size_t cbytes = dt == 0 ? 8 : 16;
       const size_t lds = (cbytes << dp.k) +
-                         (mode == 0 ? 0 : 8 * (size_t)std::max(1, dp.ngop) * (size_t)(NT / 64));
+                         (mode == 0 ? 0 : 8 * (size_t)std::max(1, dp.ngop) * (size_t)adj_partials(dt, dp));
       const int blocks_cu = (int)std::max<size_t>(1, (160u * 1024u) / lds);
       const int waves = std::max(1, std::min(8, blocks_cu * (NT / 64) / 4));
       // parameter lists: jit_launch_fwd / jit_launch_adj pass exactly these
