@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--adj-tile", type=int, default=0, help="adjoint tile bits (0 = default)")
     ap.add_argument("--jit", type=int, default=1, help="1: compiled pass kernels (pass compiler), 0: generic")
     ap.add_argument("--reg-slots", type=int, default=0, help="register-resident qubits (0 = default)")
+    ap.add_argument("--greedy", type=int, default=0,
+                    help="1: first-fit pass scheduling (no search for fewer passes)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -96,7 +98,8 @@ def main():
     ham = engine.PauliSum(n, codes, coeffs)
     t_stage = time.perf_counter()
     prog = engine.Program(n, specs, P, fuse=1, ckpt_every=args.ckpt, tile_bits=args.tile,
-                          adj_tile_bits=args.adj_tile, jit=args.jit, reg_slots=args.reg_slots)
+                          adj_tile_bits=args.adj_tile, jit=args.jit, reg_slots=args.reg_slots,
+                          greedy=args.greedy)
     jit_s = 0.0
     if args.jit:
         jit_s = prog.compile("complex64", 0) + prog.compile("complex64", 1)
@@ -145,7 +148,8 @@ def main():
         "config": {"workload": "hea_tfim_energy_grad", "nqubits": n, "layers": L, "params_per_set": P,
                    "batch": B, "ckpt_every_layers": args.ckpt, "h": 1.0, "seed": 1004,
                    "fwd_passes": prog.npasses, "staging_s": round(staging_s, 3), "jit": args.jit,
-                   "jit_compile_s": round(jit_s, 2), "reg_slots": args.reg_slots or 4},
+                   "jit_compile_s": round(jit_s, 2), "reg_slots": args.reg_slots or 4,
+                   "sched": "first-fit" if args.greedy else "search"},
     }
 
     # ------------------------------------------------------------ plain path

@@ -95,7 +95,11 @@ typedef struct tcm_opts {
                           (straight-line code, hiprtc) on first use of a
                           dtype/mode; identical passes share a kernel and
                           code objects are cached on disk */
-  int32_t reserved;
+  int32_t greedy;      /* 1: first-fit choice of each pass's tile qubits and
+                          of each register group; 0: first-fit improved by
+                          hill climbing on the number of gates a pass (or
+                          group) takes, which needs fewer passes on layered
+                          circuits */
 } tcm_opts;
 
 typedef struct tcm_program tcm_program; /* opaque, caller-owned */

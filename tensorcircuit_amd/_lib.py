@@ -44,7 +44,7 @@ class TcmOpts(C.Structure):
         ("reg_slots", C.c_int32),
         ("mfma", C.c_int32),
         ("jit", C.c_int32),
-        ("reserved", C.c_int32),
+        ("greedy", C.c_int32),
     ]
 
 

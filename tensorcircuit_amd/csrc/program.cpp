@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <functional>
 #include <sstream>
 
@@ -109,54 +110,277 @@ int layout_size(int layout) {
   }
 }
 
-// Greedy clustering with commutation by disjointness: walk `items` in
-// program order; an item joins the current cluster if its bits do not touch
-// any item deferred before it and the cluster still fits `cap` free bits.
-// Every deferred item keeps its relative order.  `solo` items form a cluster
-// of their own.
+// Clustering with commutation by disjointness.  Each cluster takes items in
+// program order: an item joins if its bits lie inside the cluster's bit set
+// (the `fixed` bits plus at most `cap` free bits) and every earlier item
+// sharing a bit with it joined too (an item that stays behind holds its
+// bits for everything after it).  Items that stay behind keep their
+// relative order.  `solo` items form a cluster of their own.
+//
+// Choice of the free-bit set: first fit (the bits of the items in the order
+// they come, while they fit), then hill climbing (swap one bit in for one
+// out, or add one while below `cap`, whenever more items join).  With
+// `climb` set, the climbed set and its best neighbours are further compared
+// by a rollout of the next few clusters under the climbing rule, and the
+// candidate that finishes the items in fewer clusters (else covers more
+// items within the rollout) wins.  On a layered circuit whose two-qubit
+// gates form a chain (hardware-efficient ansatz) first fit stalls on the
+// chain's next link, climbing alone grabs the widest window and strands the
+// lagging layers, and the rollout keeps the staircase of windows that moves
+// every layer forward together.
+struct ClusterScorer {
+  int cap = 0;
+  uint64_t fixed = 0;
+  // current item list (positions 0..G-1), its masks and predecessor lists
+  std::vector<int> rem;
+  std::vector<uint64_t> mask;
+  std::vector<char> solo;
+  std::vector<int32_t> pred_ptr, pred;
+  std::vector<uint64_t> acc;
+  std::vector<double> wt;
+  double lam = 1.0;
+
+  void set_items(const std::vector<int>& items, const std::vector<uint64_t>& gmask,
+                 const std::vector<char>& gsolo) {
+    rem = items;
+    const size_t G = rem.size();
+    mask.resize(G);
+    solo.resize(G);
+    pred_ptr.assign(G + 1, 0);
+    pred.clear();
+    int last[64];
+    for (int b = 0; b < 64; ++b) last[b] = -1;
+    for (size_t i = 0; i < G; ++i) {
+      mask[i] = gmask[rem[i]];
+      solo[i] = gsolo[rem[i]];
+      for (uint64_t t = mask[i]; t; t &= t - 1) {
+        const int b = __builtin_ctzll(t);
+        if (last[b] >= 0 && std::find(pred.begin() + pred_ptr[i], pred.end(), last[b]) == pred.end())
+          pred.push_back(last[b]);
+        last[b] = (int)i;
+      }
+      pred_ptr[i + 1] = (int32_t)pred.size();
+    }
+    acc.resize(G);
+    // weight lam^level, level = longest chain of predecessors in the list
+    std::vector<int> lev(G, 0);
+    wt.resize(G);
+    for (size_t i = 0; i < G; ++i) {
+      for (int32_t p = pred_ptr[i]; p < pred_ptr[i + 1]; ++p) lev[i] = std::max(lev[i], lev[pred[p]] + 1);
+      wt[i] = lam >= 1.0 ? 1.0 : std::pow(lam, (double)lev[i]);
+    }
+  }
+
+  // counts[c] = weight of the items taken by free-bit set sets[c], c < ns <= 64
+  void score(const uint64_t* sets, int ns, double* counts) {
+    uint64_t W[64];
+    const uint64_t all = ns >= 64 ? ~0ull : ((1ull << ns) - 1);
+    for (int b = 0; b < 64; ++b) W[b] = (fixed >> b & 1) ? all : 0;
+    for (int c = 0; c < ns; ++c)
+      for (uint64_t t = sets[c]; t; t &= t - 1) W[__builtin_ctzll(t)] |= 1ull << c;
+    for (int c = 0; c < ns; ++c) counts[c] = 0;
+    for (size_t i = 0; i < rem.size(); ++i) {
+      uint64_t a = solo[i] ? 0 : all;
+      for (uint64_t t = mask[i]; t && a; t &= t - 1) a &= W[__builtin_ctzll(t)];
+      for (int32_t p = pred_ptr[i]; p < pred_ptr[i + 1] && a; ++p) a &= acc[pred[p]];
+      acc[i] = a;
+      for (; a; a &= a - 1) counts[__builtin_ctzll(a)] += wt[i];
+    }
+  }
+
+  double count(uint64_t S) {
+    double c;
+    score(&S, 1, &c);
+    return c;
+  }
+
+  uint64_t first_fit() const {
+    uint64_t S = 0, blocked = 0;
+    for (size_t i = 0; i < rem.size(); ++i) {
+      const uint64_t m = mask[i];
+      if (solo[i] || (m & blocked)) {
+        blocked |= m;
+        continue;
+      }
+      const uint64_t ns = S | (m & ~fixed);
+      if (popc64(ns) <= cap) S = ns;
+      else blocked |= m;
+    }
+    return S;
+  }
+
+  // hill climbing from S; `near` receives the final neighbourhood (count, set)
+  uint64_t climb(uint64_t S, double* best_out, std::vector<std::pair<double, uint64_t>>* near) {
+    uint64_t cand = 0;
+    for (uint64_t m : mask) cand |= m;
+    cand &= ~fixed;
+    double best = count(S);
+    std::vector<uint64_t> nb;
+    std::vector<double> cnt;
+    for (int iter = 0; iter < 256; ++iter) {
+      nb.clear();
+      for (uint64_t outs = cand & ~S; outs; outs &= outs - 1) {
+        const uint64_t c = outs & (~outs + 1);
+        if (popc64(S) < cap) nb.push_back(S | c);
+        for (uint64_t ins = S; ins; ins &= ins - 1) nb.push_back((S & ~(ins & (~ins + 1))) | c);
+      }
+      cnt.assign(nb.size(), 0);
+      for (size_t o = 0; o < nb.size(); o += 64)
+        score(nb.data() + o, (int)std::min<size_t>(64, nb.size() - o), cnt.data() + o);
+      size_t bi = nb.size();
+      double bc = best;
+      for (size_t j = 0; j < nb.size(); ++j)
+        if (cnt[j] > bc * (1.0 + 1e-12)) bc = cnt[j], bi = j;
+      if (bi == nb.size()) {
+        if (near) {
+          near->clear();
+          for (size_t j = 0; j < nb.size(); ++j) near->push_back({cnt[j], nb[j]});
+        }
+        break;
+      }
+      best = bc;
+      S = nb[bi];
+    }
+    *best_out = best;
+    return S;
+  }
+
+  // split the current items into those set S takes and the rest
+  void split(uint64_t S, std::vector<int>& taken, std::vector<int>& rest) const {
+    const uint64_t inside = S | fixed;
+    uint64_t held = 0;
+    taken.clear();
+    rest.clear();
+    for (size_t i = 0; i < rem.size(); ++i) {
+      const uint64_t m = mask[i];
+      if (!solo[i] && !(m & held) && !(m & ~inside)) taken.push_back(rem[i]);
+      else {
+        rest.push_back(rem[i]);
+        held |= m;
+      }
+    }
+  }
+};
+
 std::vector<std::vector<int>> cluster(const std::vector<int>& items,
                                       const std::function<uint64_t(int)>& mask,
                                       uint64_t fixed, int cap,
                                       const std::function<bool(int)>& solo,
-                                      std::vector<uint64_t>* used_out) {
-  std::vector<std::vector<int>> out;
-  std::vector<int> remaining = items;
-  while (!remaining.empty()) {
-    std::vector<int> cur, deferred;
-    uint64_t S = 0, blocked = 0;
-    bool closed = false;
-    for (int it : remaining) {
-      uint64_t m = mask(it);
-      if (closed || (m & blocked)) {
-        deferred.push_back(it);
-        blocked |= m;
-        continue;
+                                      std::vector<uint64_t>* used_out, bool climb) {
+  int maxid = 0;
+  for (int it : items) maxid = std::max(maxid, it);
+  std::vector<uint64_t> gmask(items.empty() ? 0 : maxid + 1, 0);
+  std::vector<char> gsolo(gmask.size(), 0);
+  for (int it : items) {
+    gmask[it] = mask(it);
+    gsolo[it] = solo(it) ? 1 : 0;
+  }
+  ClusterScorer sc;
+  sc.cap = cap;
+  sc.fixed = fixed;
+  if (const char* e = getenv("TCM_SCHED_LAMBDA")) sc.lam = atof(e);
+  // a partial schedule: the clusters so far and the items left
+  struct Node {
+    std::vector<std::vector<int>> clusters;
+    std::vector<uint64_t> sets;
+    std::vector<int> rest;
+    int covered = 0;
+    double wsum = 0.0, score = 0.0;
+    uint64_t next = 0;                                // climbed set for `rest`
+    std::vector<std::pair<double, uint64_t>> near;    // its neighbourhood
+  };
+  // climbed set (and neighbourhood) for the first cluster of `n.rest`
+  auto prepare = [&](Node& n) {
+    n.score = n.wsum;
+    if (n.rest.empty()) return;
+    sc.set_items(n.rest, gmask, gsolo);
+    if (sc.solo[0]) {
+      n.score += 1;
+      return;
+    }
+    double best = 0;
+    n.next = sc.first_fit();
+    if (climb) n.next = sc.climb(n.next, &best, &n.near);
+    else best = sc.count(n.next);
+    n.score += best;
+  };
+  // beam width and candidates per node (pure first fit / climbing when
+  // the item list is long)
+  int beam = 8, ncand = 8;
+  if (const char* e = getenv("TCM_SCHED_BEAM")) beam = atoi(e);
+  if (const char* e = getenv("TCM_SCHED_CANDS")) ncand = atoi(e);
+  if (!climb || items.size() > 3000) beam = ncand = 1;
+  std::vector<Node> front(1);
+  front[0].rest = items;
+  prepare(front[0]);
+  std::vector<int> taken, rest;
+  while (true) {
+    for (const Node& n : front)
+      if (n.rest.empty()) {
+        if (used_out) *used_out = n.sets;
+        return n.clusters;
       }
-      if (solo(it)) {
-        if (cur.empty()) {
-          cur.push_back(it);
-          S |= (m & ~fixed);
-          closed = true;
-        } else {
-          deferred.push_back(it);
-          blocked |= m;
-        }
-        continue;
-      }
-      uint64_t ns = S | (m & ~fixed);
-      if (popc64(ns) <= cap) {
-        S = ns;
-        cur.push_back(it);
+    std::vector<Node> kids;
+    for (const Node& n : front) {
+      sc.set_items(n.rest, gmask, gsolo);
+      std::vector<uint64_t> cands;
+      if (sc.solo[0]) {
+        cands.push_back(~0ull);  // marker: the solo item alone
       } else {
-        deferred.push_back(it);
-        blocked |= m;
+        cands.push_back(n.next);
+        std::vector<std::pair<double, uint64_t>> near = n.near;
+        std::sort(near.begin(), near.end(),
+                  [](const std::pair<double, uint64_t>& a, const std::pair<double, uint64_t>& b) {
+                    return a.first > b.first || (a.first == b.first && a.second < b.second);
+                  });
+        for (const auto& pr : near) {
+          if ((int)cands.size() >= ncand) break;
+          if (pr.first > 0) cands.push_back(pr.second);
+        }
+      }
+      for (uint64_t c : cands) {
+        Node k;
+        k.clusters = n.clusters;
+        k.sets = n.sets;
+        uint64_t S;
+        k.wsum = n.wsum;
+        if (c == ~0ull) {
+          taken.assign(1, n.rest[0]);
+          rest.assign(n.rest.begin() + 1, n.rest.end());
+          S = gmask[n.rest[0]] & ~fixed;
+          k.wsum += 1;
+        } else {
+          k.wsum += sc.count(c);
+          sc.split(c, taken, rest);
+          uint64_t usedb = 0;
+          for (int it : taken) usedb |= gmask[it];
+          S = c & usedb;
+        }
+        k.clusters.push_back(taken);
+        k.sets.push_back(S);
+        k.rest = rest;
+        k.covered = n.covered + (int)taken.size();
+        kids.push_back(std::move(k));
       }
     }
-    out.push_back(cur);
-    if (used_out) used_out->push_back(S);
-    remaining.swap(deferred);
+    for (Node& k : kids) prepare(k);
+    // best `beam` distinct children: score = items covered plus what the
+    // next climbed cluster would take
+    std::stable_sort(kids.begin(), kids.end(), [](const Node& a, const Node& b) {
+      if (a.rest.empty() != b.rest.empty()) return a.rest.empty();
+      if (a.score != b.score) return a.score > b.score;
+      return a.covered > b.covered;
+    });
+    front.clear();
+    for (Node& k : kids) {
+      bool dup = false;
+      for (const Node& f : front)
+        if (f.rest == k.rest) dup = true;
+      if (dup) continue;
+      front.push_back(std::move(k));
+      if ((int)front.size() >= beam) break;
+    }
   }
-  return out;
 }
 
 uint64_t gate_mask(const LGate& g) {
@@ -486,7 +710,7 @@ static void tile_run(const Program& P, Sched& S, const std::vector<int>& items, 
   } else {
     clusters = cluster(
         items, [&](int gi) { return gate_mask(P.gates[gi]); }, lmask, cap,
-        [](int) { return false; }, &used);
+        [](int) { return false; }, &used, !P.opts.greedy);
   }
   for (size_t ci = 0; ci < clusters.size(); ++ci) {
     HPass hp;
@@ -515,7 +739,7 @@ static void tile_run(const Program& P, Sched& S, const std::vector<int>& items, 
     };
     std::vector<uint64_t> gused;
     auto groups = cluster(
-        hp.gates, lmask_of, 0, R, [&](int gi) { return P.gates[gi].nq >= 3; }, &gused);
+        hp.gates, lmask_of, 0, R, [&](int gi) { return P.gates[gi].nq >= 3; }, &gused, !P.opts.greedy);
     for (size_t q = 0; q < groups.size(); ++q) {
       HGroup hg;
       hg.gates = groups[q];

@@ -80,7 +80,8 @@ class Program:
     fused passes (1) or one kernel per gate (0, the plain path)."""
 
     def __init__(self, n: int, specs: Sequence[GateSpec], nparams: int, fuse: int = 1, tile_bits: int = 0,
-                 adj_tile_bits: int = 0, ckpt_every: int = 0, reg_slots: int = 0, mfma: int = 0, jit: int = 0):
+                 adj_tile_bits: int = 0, ckpt_every: int = 0, reg_slots: int = 0, mfma: int = 0, jit: int = 0,
+                 greedy: int = 0):
         self.n = int(n)
         self.nparams = int(nparams)
         self.specs = list(specs)
@@ -101,6 +102,7 @@ class Program:
         opts = _lib.TcmOpts()
         opts.fuse, opts.tile_bits, opts.adj_tile_bits = fuse, tile_bits, adj_tile_bits
         opts.ckpt_every, opts.reg_slots, opts.mfma, opts.jit = ckpt_every, reg_slots, mfma, jit
+        opts.greedy = greedy
         h = C.c_void_p()
         check(lib.tcm_program_build(arr, len(self.specs), self.n, self.nparams, C.byref(opts), C.byref(h)),
               "tcm_program_build")
