@@ -211,14 +211,14 @@ int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const 
   const DPass& P = S.dpass[pi];
   const JitPass& J = S.jit[pi];
   (void)dt;
-  long long sstride = 1ll << n, mstride = S.mtab_size, mbase = J.mbase;
+  long long sstride = 1ll << n, mstride = S.mtab_size, mbase = J.mbase, dbase = J.dbase;
   void* a_st = state;
   const void* a_mt = mtab;
   const void* a_ct = ctab;
   void* a_ck = ckout;
   long long total = batch << P.nblk_log;
   // one entry per parameter of the generated signature, in its order
-  void* args[] = {&a_st, &sstride, &a_mt, &mstride, &mbase, &a_ct, &a_ck, &total};
+  void* args[] = {&a_st, &sstride, &a_mt, &mstride, &mbase, &dbase, &a_ct, &a_ck, &total};
   const unsigned blocks = (unsigned)total;
   const hipError_t e = hipModuleLaunchKernel((hipFunction_t)D.jit_fn[J.kernel], blocks, 1, 1,
                                              1u << P.tthreads_log, 1, 1, 0, st, args, nullptr);
@@ -232,7 +232,7 @@ int jit_launch_adj(const Sched& S, const DeviceTables& D, int pi, int dt, const 
   const DPass& P = S.dpass[pi];
   const JitPass& J = S.jit[pi];
   (void)dt;
-  long long sstride = 1ll << n, mstride = S.mtab_size, mbase = J.mbase, ps = pstride;
+  long long sstride = 1ll << n, mstride = S.mtab_size, mbase = J.mbase, dbase = J.dbase, ps = pstride;
   const void* a_pi = psi_in;
   void* a_po = psi_out;
   const void* a_li = lam_in;
@@ -241,7 +241,7 @@ int jit_launch_adj(const Sched& S, const DeviceTables& D, int pi, int dt, const 
   const void* a_ct = ctab;
   double* a_part = part;
   long long total = batch << P.nblk_log;
-  void* args[] = {&a_pi, &a_po, &a_li, &a_lo, &sstride, &a_mt, &mstride, &mbase, &a_ct, &a_part, &ps, &total};
+  void* args[] = {&a_pi, &a_po, &a_li, &a_lo, &sstride, &a_mt, &mstride, &mbase, &dbase, &a_ct, &a_part, &ps, &total};
   const unsigned blocks = (unsigned)total;
   const hipError_t e = hipModuleLaunchKernel((hipFunction_t)D.jit_fn[J.kernel], blocks, 1, 1,
                                              1u << P.tthreads_log, 1, 1, 0, st, args, nullptr);

@@ -70,13 +70,18 @@ __device__ __forceinline__ unsigned swz(unsigned t) {
 __device__ __forceinline__ T imcj(cT a, cT b) { return a.x * b.y - a.y * b.x; }
 __device__ __forceinline__ T recj(cT a, cT b) { return a.x * b.x + a.y * b.y; }
 union tcm_d2i { double d; int i[2]; };
+// row_shr moves inside each 16-lane row; lanes with no source read 0
 template <int CTRL>
 __device__ __forceinline__ double dpp_shr(double x) {
   tcm_d2i u, r;
   u.d = x;
-  r.i[0] = __builtin_amdgcn_update_dpp(0, u.i[0], CTRL, 0xF, 0xF, false);
-  r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], CTRL, 0xF, 0xF, false);
+  r.i[0] = __builtin_amdgcn_update_dpp(0, u.i[0], CTRL, 0xF, 0xF, true);
+  r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], CTRL, 0xF, 0xF, true);
   return r.d;
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double read_lane(double x, int lane) {
   tcm_d2i u, r;
@@ -85,14 +90,18 @@ __device__ __forceinline__ double read_lane(double x, int lane) {
   r.i[1] = __builtin_amdgcn_readlane(u.i[1], lane);
   return r.d;
 }
-// fp64 sum over each 16-lane row: the row total lands in its last lane
-__device__ __forceinline__ double rowsum(double x) {
+// sum over each 16-lane row in the working precision (the row total lands
+// in its last lane); a float row sum covers 256 amplitudes (16 per thread)
+// and is widened to double by the caller
+template <typename V>
+__device__ __forceinline__ V rowsum(V x) {
   x += dpp_shr<0x111>(x);
   x += dpp_shr<0x112>(x);
   x += dpp_shr<0x114>(x);
   x += dpp_shr<0x118>(x);
   return x;
 }
+// row_bcast into the rows of ROWS; the other rows add the zero `old` value
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_rows(double x) {
   tcm_d2i u, r;
@@ -101,14 +110,16 @@ __device__ __forceinline__ double dpp_rows(double x) {
   r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], CTRL, ROWS, 0xF, false);
   return r.d;
 }
-// fp64 sum over the wave, total in lane 63 only: row prefix sums by
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_rows(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
+}
+// sum over the wave, total in lane 63 only: row prefix sums by
 // row_shr 1/2/4/8, then row_bcast:15 into rows 1 and 3 and row_bcast:31
-// into rows 2 and 3 (other rows add the zero `old` value)
-__device__ __forceinline__ double wsum63(double x) {
-  x += dpp_shr<0x111>(x);
-  x += dpp_shr<0x112>(x);
-  x += dpp_shr<0x114>(x);
-  x += dpp_shr<0x118>(x);
+// into rows 2 and 3
+template <typename V>
+__device__ __forceinline__ V wsum63(V x) {
+  x = rowsum(x);
   x += dpp_rows<0x142, 0xA>(x);
   x += dpp_rows<0x143, 0xC>(x);
   return x;
@@ -129,19 +140,23 @@ struct Gen {
   const HPass& hp;
   bool dag;       // adjoint: apply U^dagger, psi and lambda arrays
   int R, nreg;
-  int mbase;
+  int mbase, dbase;
   std::string out;
   std::vector<int> where;  // logical register index -> variable index
 
   Gen(const Sched& s, const DPass& d, const HPass& h, bool adj) : S(s), dp(d), hp(h), dag(adj), R(s.R) {
     nreg = 1 << R;
-    mbase = -1;
+    mbase = dbase = -1;
   }
 
   void line(const std::string& s) { out += s; out += '\n'; }
 
-  // matrix element source for an op's table offset `moff` (+i)
+  // matrix element source for an op's table offset `moff` (+i): base
+  // recipes relative to mbase, merged diagonal tables (built per schedule
+  // after the base recipes) relative to dbase, so that passes repeating
+  // with shifted tables share one kernel
   std::string mel(int moff, int i) const {
+    if (moff >= S.mtab_base) return fmt("md[%d]", moff - dbase + i);
     if (moff >= 0) return fmt("mt[%d]", moff - mbase + i);
     return fmt("ct[%d]", -moff - 1 + i);
   }
@@ -421,9 +436,9 @@ bool pass_supported(const Sched& S, const DPass& dp) {
   return true;
 }
 
-// body of one pass kernel (everything after the signature); mbase is the
-// smallest per-item table offset used by the pass
-std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt, int* mbase_out) {
+// body of one pass kernel (everything after the signature); mbase / dbase
+// are the smallest base-recipe / merged-diagonal table offsets it reads
+std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt, int* mbase_out, int* dbase_out) {
   const DPass& dp = S.dpass[pi];
   const HPass& hp = S.passes[pi];
   Gen G(S, dp, hp, adj);
@@ -439,17 +454,22 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   const uint32_t lmask = rows ? 15u : 63u;
   const int lshift = rows ? 4 : 6;
   const char* IT = wide ? "unsigned long long" : "unsigned";
-  int mb = -1;
+  int mb = -1, db = -1;
   for (int g = 0; g < dp.ngroups; ++g) {
     const DGroup& dg = S.dgroup[dp.group0 + g];
-    for (int o = dg.op0; o < dg.op0 + dg.nops; ++o)
-      if (S.dop[o].moff >= 0 && (S.dop[o].kind != OK_X1 && S.dop[o].kind != OK_CX && S.dop[o].kind != OK_CZ &&
-                                 S.dop[o].kind != OK_SWAP))
-        mb = mb < 0 ? S.dop[o].moff : std::min(mb, S.dop[o].moff);
+    for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) {
+      const DOp& op = S.dop[o];
+      if (op.moff < 0 || op.kind == OK_X1 || op.kind == OK_CX || op.kind == OK_CZ || op.kind == OK_SWAP) continue;
+      int& base = op.moff >= S.mtab_base ? db : mb;
+      base = base < 0 ? op.moff : std::min(base, op.moff);
+    }
   }
   if (mb < 0) mb = 0;
+  if (db < 0) db = S.mtab_base;
   G.mbase = mb;
+  G.dbase = db;
   *mbase_out = mb;
+  *dbase_out = db;
 
   // block base and thread / element deposits
   std::vector<int> bs, bd, ts, td;
@@ -475,6 +495,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   G.line(fmt("%s dlo = %s;", IT, deposit_expr("tid", ts, td, IT).c_str()));
   G.line("unsigned slo = swz(tid) * (unsigned)sizeof(cT);");
   G.line("const T* __restrict__ mt = mtab + b * mstride + mbase;");
+  G.line("const T* __restrict__ md = mtab + b * mstride + dbase;");
   const int nit = 1 << (k - tl);  // elements per thread
   auto hi = [&](int it) { return (unsigned long long)dep_const((uint64_t)it << tl); };
 
@@ -498,14 +519,14 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
               if (sg) e += fmt(" %s im%d", sg > 0 ? "+" : "-", j);
             }
             if (e.empty()) e = " + 0";
-            G.line(fmt("{ const T ga = (T)0%s; const double gs = %s((double)ga);", e.c_str(), rsum));
+            G.line(fmt("{ const T ga = (T)0%s; const double gs = (double)%s(ga);", e.c_str(), rsum));
             G.line(fmt("  if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs; }", lmask, lmask, M.gloc * PR, lshift));
           }
           G.line("}");
         } else if (op.gen != GN_NONE) {
           G.line("{");
           if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return false;
-          G.line(fmt("const double gs = %s((double)ga);", rsum));
+          G.line(fmt("const double gs = (double)%s(ga);", rsum));
           G.line(fmt("if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs;", lmask, lmask, op.gloc * PR, lshift));
           G.line("}");
         }
@@ -655,8 +676,8 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
   for (int pi = 0; pi < (int)S.dpass.size(); ++pi) {
     const DPass& dp = S.dpass[pi];
     if (!pass_supported(S, dp)) continue;
-    int mb = 0;
-    std::string body = pass_body(P, S, pi, mode == 1, dt, &mb);
+    int mb = 0, db = 0;
+    std::string body = pass_body(P, S, pi, mode == 1, dt, &mb, &db);
     if (body.empty()) continue;
     const int NT = 1 << dp.tthreads_log;
     const std::string key = fmt("%d|", NT) + body;
@@ -677,14 +698,14 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
       if (mode == 0)
         k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "
                 "tcm_fwd_k%d(cT* __restrict__ st, long long sstride, "
-                "const T* __restrict__ mtab, long long mstride, long long mbase, const T* __restrict__ ct, "
-                "cT* __restrict__ ck, long long total) {\n",
+                "const T* __restrict__ mtab, long long mstride, long long mbase, long long dbase, "
+                "const T* __restrict__ ct, cT* __restrict__ ck, long long total) {\n",
                 NT, waves, id);
       else
         k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "
                 "tcm_adj_k%d(const cT* psi_in, "
                 "cT* psi_out, const cT* lam_in, cT* lam_out, "
-                "long long sstride, const T* __restrict__ mtab, long long mstride, long long mbase, "
+                "long long sstride, const T* __restrict__ mtab, long long mstride, long long mbase, long long dbase, "
                 "const T* __restrict__ ct, double* __restrict__ part, long long pstride, long long total) {\n",
                 NT, waves, id);
       bodies.push_back(k + body + "}\n");
@@ -693,6 +714,7 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
     }
     jp[pi].kernel = id;
     jp[pi].mbase = mb;
+    jp[pi].dbase = db;
   }
   src = std::string("#define REAL ") + (dt == 0 ? "float" : "double") + "\n" + kPrelude;
   return src;

@@ -137,8 +137,6 @@ struct ClusterScorer {
   std::vector<char> solo;
   std::vector<int32_t> pred_ptr, pred;
   std::vector<uint64_t> acc;
-  std::vector<double> wt;
-  double lam = 1.0;
 
   void set_items(const std::vector<int>& items, const std::vector<uint64_t>& gmask,
                  const std::vector<char>& gsolo) {
@@ -162,17 +160,10 @@ struct ClusterScorer {
       pred_ptr[i + 1] = (int32_t)pred.size();
     }
     acc.resize(G);
-    // weight lam^level, level = longest chain of predecessors in the list
-    std::vector<int> lev(G, 0);
-    wt.resize(G);
-    for (size_t i = 0; i < G; ++i) {
-      for (int32_t p = pred_ptr[i]; p < pred_ptr[i + 1]; ++p) lev[i] = std::max(lev[i], lev[pred[p]] + 1);
-      wt[i] = lam >= 1.0 ? 1.0 : std::pow(lam, (double)lev[i]);
-    }
   }
 
-  // counts[c] = weight of the items taken by free-bit set sets[c], c < ns <= 64
-  void score(const uint64_t* sets, int ns, double* counts) {
+  // counts[c] = number of items taken by free-bit set sets[c], c < ns <= 64
+  void score(const uint64_t* sets, int ns, int* counts) {
     uint64_t W[64];
     const uint64_t all = ns >= 64 ? ~0ull : ((1ull << ns) - 1);
     for (int b = 0; b < 64; ++b) W[b] = (fixed >> b & 1) ? all : 0;
@@ -184,12 +175,12 @@ struct ClusterScorer {
       for (uint64_t t = mask[i]; t && a; t &= t - 1) a &= W[__builtin_ctzll(t)];
       for (int32_t p = pred_ptr[i]; p < pred_ptr[i + 1] && a; ++p) a &= acc[pred[p]];
       acc[i] = a;
-      for (; a; a &= a - 1) counts[__builtin_ctzll(a)] += wt[i];
+      for (; a; a &= a - 1) ++counts[__builtin_ctzll(a)];
     }
   }
 
-  double count(uint64_t S) {
-    double c;
+  int count(uint64_t S) {
+    int c;
     score(&S, 1, &c);
     return c;
   }
@@ -210,13 +201,13 @@ struct ClusterScorer {
   }
 
   // hill climbing from S; `near` receives the final neighbourhood (count, set)
-  uint64_t climb(uint64_t S, double* best_out, std::vector<std::pair<double, uint64_t>>* near) {
+  uint64_t climb(uint64_t S, int* best_out, std::vector<std::pair<int, uint64_t>>* near) {
     uint64_t cand = 0;
     for (uint64_t m : mask) cand |= m;
     cand &= ~fixed;
-    double best = count(S);
+    int best = count(S);
     std::vector<uint64_t> nb;
-    std::vector<double> cnt;
+    std::vector<int> cnt;
     for (int iter = 0; iter < 256; ++iter) {
       nb.clear();
       for (uint64_t outs = cand & ~S; outs; outs &= outs - 1) {
@@ -228,9 +219,9 @@ struct ClusterScorer {
       for (size_t o = 0; o < nb.size(); o += 64)
         score(nb.data() + o, (int)std::min<size_t>(64, nb.size() - o), cnt.data() + o);
       size_t bi = nb.size();
-      double bc = best;
+      int bc = best;
       for (size_t j = 0; j < nb.size(); ++j)
-        if (cnt[j] > bc * (1.0 + 1e-12)) bc = cnt[j], bi = j;
+        if (cnt[j] > bc) bc = cnt[j], bi = j;
       if (bi == nb.size()) {
         if (near) {
           near->clear();
@@ -278,27 +269,25 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
   ClusterScorer sc;
   sc.cap = cap;
   sc.fixed = fixed;
-  if (const char* e = getenv("TCM_SCHED_LAMBDA")) sc.lam = atof(e);
   // a partial schedule: the clusters so far and the items left
   struct Node {
     std::vector<std::vector<int>> clusters;
     std::vector<uint64_t> sets;
     std::vector<int> rest;
-    int covered = 0;
-    double wsum = 0.0, score = 0.0;
-    uint64_t next = 0;                                // climbed set for `rest`
-    std::vector<std::pair<double, uint64_t>> near;    // its neighbourhood
+    int covered = 0, score = 0;
+    uint64_t next = 0;                             // climbed set for `rest`
+    std::vector<std::pair<int, uint64_t>> near;    // its neighbourhood
   };
   // climbed set (and neighbourhood) for the first cluster of `n.rest`
   auto prepare = [&](Node& n) {
-    n.score = n.wsum;
+    n.score = n.covered;
     if (n.rest.empty()) return;
     sc.set_items(n.rest, gmask, gsolo);
     if (sc.solo[0]) {
       n.score += 1;
       return;
     }
-    double best = 0;
+    int best = 0;
     n.next = sc.first_fit();
     if (climb) n.next = sc.climb(n.next, &best, &n.near);
     else best = sc.count(n.next);
@@ -307,8 +296,6 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
   // beam width and candidates per node (pure first fit / climbing when
   // the item list is long)
   int beam = 8, ncand = 8;
-  if (const char* e = getenv("TCM_SCHED_BEAM")) beam = atoi(e);
-  if (const char* e = getenv("TCM_SCHED_CANDS")) ncand = atoi(e);
   if (!climb || items.size() > 3000) beam = ncand = 1;
   std::vector<Node> front(1);
   front[0].rest = items;
@@ -328,9 +315,9 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
         cands.push_back(~0ull);  // marker: the solo item alone
       } else {
         cands.push_back(n.next);
-        std::vector<std::pair<double, uint64_t>> near = n.near;
+        std::vector<std::pair<int, uint64_t>> near = n.near;
         std::sort(near.begin(), near.end(),
-                  [](const std::pair<double, uint64_t>& a, const std::pair<double, uint64_t>& b) {
+                  [](const std::pair<int, uint64_t>& a, const std::pair<int, uint64_t>& b) {
                     return a.first > b.first || (a.first == b.first && a.second < b.second);
                   });
         for (const auto& pr : near) {
@@ -343,14 +330,11 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
         k.clusters = n.clusters;
         k.sets = n.sets;
         uint64_t S;
-        k.wsum = n.wsum;
         if (c == ~0ull) {
           taken.assign(1, n.rest[0]);
           rest.assign(n.rest.begin() + 1, n.rest.end());
           S = gmask[n.rest[0]] & ~fixed;
-          k.wsum += 1;
         } else {
-          k.wsum += sc.count(c);
           sc.split(c, taken, rest);
           uint64_t usedb = 0;
           for (int it : taken) usedb |= gmask[it];
@@ -589,6 +573,7 @@ static void base_recipes(const Program& P, std::vector<DRecipe>& out, int* size)
 static void sched_plain(const Program& P, Sched& S) {
   S.k = 0;
   base_recipes(P, S.recipes, &S.mtab_size);
+  S.mtab_base = S.mtab_size;
   size_t ci = 0;
   for (int gi = 0; gi < (int)P.gates.size(); ++gi) {
     while (ci < P.ckpt_gate.size() && P.ckpt_gate[ci] == gi) {
@@ -771,6 +756,7 @@ static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant) {
   S.R = R;
   S.Lwant = std::min(Lwant, k);
   base_recipes(P, S.recipes, &S.mtab_size);
+  S.mtab_base = S.mtab_size;
   const uint64_t lmask = (S.Lwant >= 64) ? ~0ull : ((1ull << S.Lwant) - 1);
   const int cap = k - S.Lwant;
   // segments between checkpoints

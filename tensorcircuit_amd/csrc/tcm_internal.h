@@ -163,7 +163,8 @@ struct DeviceTables;  // per-device uploaded tables (kernels.hip)
 // generic fused kernel runs it) and the pass's per-item table offset base.
 struct JitPass {
   int32_t kernel = -1;
-  int32_t mbase = 0;
+  int32_t mbase = 0;  // smallest base-recipe offset the pass reads
+  int32_t dbase = 0;  // smallest merged-diagonal table offset it reads
 };
 
 // One schedule: a sequence of passes for one dtype and one mode (forward /
@@ -184,6 +185,8 @@ struct Sched {
   std::vector<DRecipe> recipes; // base recipes + DG recipes of this schedule
   std::vector<DFactor> factors;
   int32_t mtab_size = 0;        // per-batch matrix table size (T units)
+  int32_t mtab_base = 0;        // end of the base recipes; merged diagonal
+                                // tables of this schedule follow
   int32_t max_ngop_pass = 0;    // largest number of gradient ops in a pass
   int32_t max_slots = 0;        // largest number of gradient partial slots per gop
   std::vector<int32_t> gop_slots;  // per gop: number of partial slots (blocks)
