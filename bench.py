@@ -26,10 +26,18 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per adjoint pass launch measured with rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE on the default workload (24q, 20 layers, batch 32, checkpoints
 # every 5 layers, default tiles), FETCH scaled x2.0 and WRITE x1.0 by a 4 GiB
-# copy in the same run.  Key: (qubits, layers, batch, ckpt, jit).  Commands
-# and per-counter figures: profiles/tcm_adj_passes_hea24_b32.md (jit=1) and
-# profiles/k_fused_adj_hea24_b32.md (jit=0).
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0): 17046845037, (24, 20, 32, 5, 1): 17056017082}
+# copy in the same run.  Key: (qubits, layers, batch, ckpt, jit, first-fit
+# scheduling).  Commands and per-counter figures:
+# profiles/tcm_adj_passes_hea24_b32.md (compiled passes, searched schedule)
+# and profiles/k_fused_adj_hea24_b32.md (generic kernels, first-fit schedule).
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 18065808179}
+# VALU wave-instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, mean over
+# the adjoint / forward pass launches of one sweep), same workload and key;
+# profiles/tcm_adj_passes_hea24_b32.md.
+MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (3748410163, 1334339174)}
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
+# every 4 cycles at 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 def parse():
@@ -203,8 +211,11 @@ def main():
         adj_gbs = adj_bytes / (adj_launch_ms * 1e-3) / 1e9
         fwd_gbs = fwd_bytes / (fwd_launch_ms * 1e-3) / 1e9
         traffic = None
-        if args.adj_tile == 0 and args.reg_slots in (0, 4):
-            traffic = MEASURED_ADJ_TRAFFIC.get((n, L, B, args.ckpt, args.jit))
+        valu = None
+        if args.adj_tile == 0 and args.tile == 0 and args.reg_slots in (0, 4):
+            key = (n, L, B, args.ckpt, args.jit, args.greedy)
+            traffic = MEASURED_ADJ_TRAFFIC.get(key)
+            valu = MEASURED_VALU_INSTS.get(key)
         kname = "tcm_adj_k* (compiled adjoint passes)" if args.jit else "k_fused_adj"
         out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": round(adj_gbs, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
@@ -214,6 +225,16 @@ def main():
                                "achieved": round(fwd_gbs, 1), "unit": "GB/s",
                                "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": fwd_bytes,
                                "avg_launch_ms": round(fwd_launch_ms, 4), "launches_per_step": n_fwd}
+        if valu is not None:
+            # the compiled passes issue more VALU work per byte than HBM can
+            # feed when many gates share a pass: their binding roof is VALU issue
+            va = valu[0] / (adj_launch_ms * 1e-3)
+            vf = valu[1] / (fwd_launch_ms * 1e-3)
+            out["roofline_valu"] = {"bound": "valu", "kernel": kname, "achieved": round(va / 1e9, 1),
+                                    "peak": round(VALU_PEAK_WAVE_INSTS / 1e9, 1), "unit": "G wave-instr/s",
+                                    "frac": round(va / VALU_PEAK_WAVE_INSTS, 4), "insts_per_launch": valu[0],
+                                    "fwd_achieved": round(vf / 1e9, 1),
+                                    "fwd_frac": round(vf / VALU_PEAK_WAVE_INSTS, 4)}
         print("fused fwd pass %.3f ms (%.0f GB/s), adjoint pass %.3f ms (%.0f GB/s)"
               % (fwd_launch_ms, fwd_gbs, adj_launch_ms, adj_gbs), file=sys.stderr)
 
@@ -321,15 +342,22 @@ def run_c3(args):
         k = min(args.tile, n) if args.tile else engine.autotune_tile(n, specs, len(thetas), batch)
         return engine.Program(n, specs, len(thetas), fuse=1, tile_bits=k), len(thetas)
 
+    def hamiltonian(n):
+        # compiled Pauli passes (one kernel per tile pass, compile time is
+        # part of staging)
+        h = engine.PauliSum(n, *workloads.tfim_terms(n), jit=1)
+        h.compile("complex64")
+        return h
+
     prog16, P16 = program(16, 8, 1024)
     pa = torch.as_tensor(rng.uniform(0, 2 * np.pi, size=(1024, P16)).astype(np.float32)).cuda()
     items.append((prog16, pa, torch.empty(1024, 1 << 16, dtype=torch.complex64, device="cuda")))
-    hams.append(engine.PauliSum(16, *workloads.tfim_terms(16)))
+    hams.append(hamiltonian(16))
     for n in sorted(groups):
         prog, P = program(n, 6, len(groups[n]))
         pb = torch.as_tensor(rng.uniform(0, 2 * np.pi, size=(len(groups[n]), P)).astype(np.float32)).cuda()
         items.append((prog, pb, torch.empty(len(groups[n]), 1 << n, dtype=torch.complex64, device="cuda")))
-        hams.append(engine.PauliSum(n, *workloads.tfim_terms(n)))
+        hams.append(hamiltonian(n))
     staging = time.perf_counter() - t_stage
 
     def step():
