@@ -198,6 +198,46 @@ int tcm_inner(int32_t nqubits, int32_t dtype, const void* a, const void* b,
               int64_t batch, double* out, void* workspace, int64_t ws_bytes,
               void* stream);
 
+/* ---- measurement ----------------------------------------------------------
+ * out[b][i] = |psi_b[i]|^2 in the state's real type (float for complex64):
+ * Circuit.probability (basecircuit.py:510-524). */
+int tcm_probability(int32_t nqubits, int32_t dtype, const void* state,
+                    int64_t batch, void* out, void* stream);
+
+/* Draws from p_i = |psi_b[i]|^2 by inverse CDF, the reference's
+ * backend.probability_sample (abstract_backend.py:1124-1157) behind
+ * Circuit.sample(allow_state=True) and quantum.measurement_counts
+ * (basecircuit.py:586-604, quantum.py:2218-2322): for uniforms u (device
+ * fp64 [batch][shots], each in [0, 1]) r = total * (1 - u) and
+ * out[b][s] (device int64) is the smallest index whose inclusive prefix sum
+ * of p reaches r.  Prefix sums in fp64 (the reference sums the normalised
+ * vector in the state dtype).  Workspace: tcm_sample_workspace_bytes. */
+int tcm_sample_workspace_bytes(int32_t nqubits, int64_t batch, int64_t* bytes);
+int tcm_sample(int32_t nqubits, int32_t dtype, const void* state, int64_t batch,
+               const double* uniforms, int64_t shots, int64_t* out,
+               void* workspace, int64_t ws_bytes, void* stream);
+/* The same draws from probability vectors p (device [batch][2^nqubits],
+ * float for dtype TCM_C64, double for TCM_C128), normalised by their sum:
+ * backend.probability_sample on a given p (measurement_counts with
+ * is_prob=True, quantum.py:2290-2292). */
+int tcm_sample_prob(int32_t nqubits, int32_t dtype, const void* p, int64_t batch,
+                    const double* uniforms, int64_t shots, int64_t* out,
+                    void* workspace, int64_t ws_bytes, void* stream);
+
+/* Sequential measurement of `qubits` (host int32 [nmeas], distinct, in
+ * order) as measure_jit (basecircuit.py:371-445) with external randomness
+ * status (device fp64 [batch][nmeas]): at step k, pu = P(qubit k = 0 and
+ * the outcomes so far) / p, outcome 1 iff status - pu + 0.31415926e-12 > 0,
+ * p *= pu (outcome 0) or 1 - pu, p starting at 1.  outcomes: device fp64
+ * [batch][nmeas] (0.0 / 1.0); prob: device fp64 [batch] = final p.
+ * perfect_sampling (basecircuit.py:359-369) is all qubits in order.
+ * Workspace: tcm_measure_workspace_bytes. */
+int tcm_measure_workspace_bytes(int32_t nqubits, int64_t batch, int64_t* bytes);
+int tcm_measure(int32_t nqubits, int32_t dtype, const void* state, int64_t batch,
+                const int32_t* qubits, int32_t nmeas, const double* status,
+                double* outcomes, double* prob, void* workspace,
+                int64_t ws_bytes, void* stream);
+
 /* ---- gradients ------------------------------------------------------------
  * Replaces backend.value_and_grad / vvag over an energy function
  * (backends/pytorch_backend.py:497-544, 673-709): one forward, E and

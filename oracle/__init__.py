@@ -12,6 +12,9 @@ Contents
                  ``expectation_ps`` / Pauli sums, the HEA/TFIM workloads,
                  an adjoint gradient, central finite differences and the
                  parameter-shift rule.
+``sampling``     float64 restatements of the reference's measurement
+                 algorithms: probability_sample, measure_jit, correlations,
+                 readout-error matrices.
 ``plain_torch``  the reference's pytorch-backend plain method on the CPU
                  (gate-by-gate tensordot + torch autograd); used only as the
                  timed CPU baseline.
@@ -31,6 +34,10 @@ Pinning (what checks each function; details in ``tests/test_oracle.py``)
 * ``tfim`` energies: exact diagonalisation lower bound at n <= 10, all-zero
   angles give E = -(n-1) exactly.
 * ``haar_su2`` / ``brickwork_ops``: unitarity and det = 1 of every draw.
+* ``sampling``: the reference's KATs (quantum.py:2432-2436,
+  tests/test_backends.py:278-281, tests/test_circuit.py:1384-1388),
+  brute-force prefix sums, P(full record) = |psi_idx|^2, basis states,
+  Kronecker readout matrices, draw frequencies (tests/test_sampling.py).
 * ``plain_torch``: energy and autograd gradient equal ``statevector``'s
   adjoint at complex128 (5 qubits, 2 layers) to 1e-12.
 """

@@ -97,6 +97,12 @@ SIGNATURES = {
     "tcm_apply_pauli_sum": (C.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp]),
     "tcm_inner_workspace_bytes": (C.c_int, [_i32, _i64, C.POINTER(_i64)]),
     "tcm_inner": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tcm_probability": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _vp]),
+    "tcm_sample_workspace_bytes": (C.c_int, [_i32, _i64, C.POINTER(_i64)]),
+    "tcm_sample": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tcm_sample_prob": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tcm_measure_workspace_bytes": (C.c_int, [_i32, _i64, C.POINTER(_i64)]),
+    "tcm_measure": (C.c_int, [_i32, _i32, _vp, _i64, C.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _i64, _vp]),
     "tcm_adjoint_workspace_bytes": (C.c_int, [_vp, _vp, _i32, _i64, _i32, C.POINTER(_i64)]),
     "tcm_adjoint_energy_grad": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "tcm_adjoint_vjp": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),

@@ -211,6 +211,40 @@ class PyTorchBackend:
             shape = (shape,)
         return low + (high - low) * torch.rand(tuple(shape), dtype=getattr(torch, dtype))
 
+    # ---- randomness and sampling (abstract_backend.py:1124-1157) ----------
+    def get_random_state(self, seed: Optional[int] = None) -> torch.Generator:
+        g = torch.Generator()
+        if seed is not None:
+            g.manual_seed(int(seed))
+        else:
+            g.seed()
+        return g
+
+    def stateful_randu(self, g: torch.Generator, shape: Union[int, Sequence[int]] = 1, low: float = 0.0,
+                       high: float = 1.0, dtype: str = "float32") -> Tensor:
+        if isinstance(shape, int):
+            shape = (shape,)
+        return low + (high - low) * torch.rand(tuple(shape), generator=g, dtype=getattr(torch, dtype))
+
+    def probability_sample(self, shots: int, p: Tensor, status: Optional[Tensor] = None, g: Any = None) -> Tensor:
+        """``shots`` indices drawn from p (normalised) by inverse CDF on the
+        device: r = total * (1 - status), index = smallest i whose prefix sum
+        reaches r.  p of any length (zero-padded to a power of two)."""
+        from . import engine
+
+        pt = torch.as_tensor(p).reshape(-1)
+        if pt.is_complex():
+            pt = torch.real(pt)
+        d = int(pt.shape[0])
+        n = max(1, (d - 1).bit_length())
+        if d < (1 << n):
+            pt = torch.cat([pt, torch.zeros((1 << n) - d, dtype=pt.dtype, device=pt.device)])
+        if status is not None:
+            u = torch.as_tensor(status, dtype=torch.float64).reshape(1, -1)
+        else:
+            u = torch.rand((1, int(shots)), dtype=torch.float64, generator=g)
+        return engine.sample_prob_indices(pt, n, u)[0]
+
     def __repr__(self) -> str:
         return "pytorch_backend"
 

@@ -293,8 +293,118 @@ class Circuit:
         return s[(bits.to(torch.int64) * weights).sum()]
 
     def probability(self) -> Tensor:
-        """basecircuit.py:510-523."""
-        return torch.abs(self._state()) ** 2
+        """|psi|^2 over the computational basis (basecircuit.py:510-524),
+        differentiable."""
+        return engine.ProbFn.apply(self._state(), self._nstate)
+
+    # ------------------------------------------------------- measurement --
+    def measure_jit(self, *index: int, with_prob: bool = False, status: Optional[Tensor] = None
+                    ) -> Tuple[Tensor, Tensor]:
+        """Measure the given qubits one after another in the Z basis without
+        collapsing the circuit (basecircuit.py:371-445): outcome k is 1 iff
+        status[k] > P(0 | earlier outcomes) - 0.31415926e-12.  Returns the
+        outcomes (float 0/1) and, with ``with_prob``, the probability of the
+        record (else -1.0)."""
+        idx = [int(i) if int(i) >= 0 else self._nqubits + int(i) for i in index]
+        if status is None:
+            u = torch.rand(len(idx), dtype=torch.float64)
+        else:
+            u = torch.as_tensor(status, dtype=torch.float64).reshape(-1)[: len(idx)]
+        s = self._state()
+        outc, prob = engine.measure(s.detach(), self._nstate, idx, u.reshape(1, -1))
+        rdt = s.real.dtype
+        sample = outc[0].to(rdt)
+        if with_prob:
+            return sample, prob[0].to(rdt)
+        return sample, -1.0
+
+    measure = measure_jit
+
+    def perfect_sampling(self, status: Optional[Tensor] = None) -> Tuple[Tensor, Tensor]:
+        """All qubits measured in order, with the record's probability
+        (basecircuit.py:359-369)."""
+        return self.measure_jit(*range(self._nqubits), with_prob=True, status=status)
+
+    def readouterror_bs(self, readout_error: Optional[Sequence[Any]] = None, p: Optional[Tensor] = None) -> Tensor:
+        """Noisy bitstring probabilities: qubit i's readout matrix
+        [[e_i0, 1 - e_i1], [1 - e_i0, e_i1]] applied to p (basecircuit.py:760-803)."""
+        n = len(readout_error)  # type: ignore
+        q = torch.as_tensor(p)
+        q = (torch.real(q) if q.is_complex() else q).to(torch.float64).reshape(-1)
+        for i in range(n):
+            e0, e1 = float(readout_error[i][0]), float(readout_error[i][1])  # type: ignore
+            m = torch.tensor([[e0, 1 - e1], [1 - e0, e1]], dtype=torch.float64, device=q.device)
+            v = q.reshape(2**i, 2, -1)
+            q = torch.einsum("ab,ibj->iaj", m, v).reshape(-1)
+        return q
+
+    def sample(self, batch: Optional[int] = None, allow_state: bool = False,
+               readout_error: Optional[Sequence[Any]] = None, format: Optional[str] = None,
+               random_generator: Any = None, status: Optional[Tensor] = None, **kws: Any) -> Any:
+        """Bitstring samples of the output state (basecircuit.py:526-616).
+        Every draw comes from |psi|^2 on the device: with ``allow_state``
+        (and for ``batch`` draws without it) by inverse CDF over the prefix
+        sums (the reference's probability_sample), else one sequential
+        qubit-by-qubit draw (perfect_sampling); the reference's perfect
+        sampling samples the same distribution through a different map from
+        random numbers to outcomes.  ``format`` as in
+        quantum.measurement_counts; ``status`` [batch] uniforms replace the
+        generator."""
+        from . import quantum as qu
+
+        if format is None and "format_" in kws:
+            format = kws.pop("format_")
+        n = self._nqubits
+        if not allow_state and batch is None and readout_error is None:
+            u = qu._uniforms((n,), None if status is None else status, random_generator)
+            r = self.perfect_sampling(u)
+            if format is None:
+                return r
+            ch = qu.sample_bin2int(r[0].to(torch.int64)[None, :], n)
+            return qu.sample2all(sample=ch, n=n, format=format, jittable=True)
+        nb = 1 if batch is None else int(batch)
+        p = self.probability().detach()
+        if readout_error is not None:
+            p = self.readouterror_bs(readout_error, p).to(p.dtype)
+        u = qu._uniforms((1, nb), status, random_generator)
+        ch = engine.sample_prob_indices(p.reshape(1, -1), n, u)[0]
+        if format is None:
+            confg = qu.sample_int2bin(ch, n)
+            prob = (p / torch.sum(p))[ch]
+            r = list(zip(confg, prob))
+            return r[0] if batch is None else r
+        return qu.sample2all(sample=ch, n=n, format=format, jittable=True)
+
+    def sample_expectation_ps(self, x: Optional[Sequence[int]] = None, y: Optional[Sequence[int]] = None,
+                              z: Optional[Sequence[int]] = None, shots: Optional[int] = None,
+                              random_generator: Any = None, status: Optional[Tensor] = None,
+                              readout_error: Optional[Sequence[Any]] = None, noise_conf: Any = None,
+                              nmc: int = 1000, statusc: Any = None, **kws: Any) -> Tensor:
+        """Pauli-string expectation from Z-basis measurement after basis
+        rotation (H for x, rx(pi/2) for y; basecircuit.py:618-758): exact
+        from the probabilities when ``shots`` is None, else the mean parity
+        of ``shots`` device draws."""
+        from . import quantum as qu
+
+        if noise_conf is not None:
+            raise NotImplementedError("noisy sampling is outside this engine")
+        x, y, z = list(x or []), list(y or []), list(z or [])
+        c = type(self)(self._nqubits, inputs=self._state())
+        for i in x:
+            c.h(i)
+        for i in y:
+            c.rx(i, theta=np.pi / 2)
+        p = c.probability()
+        if readout_error is not None:
+            p = self.readouterror_bs(readout_error, p).to(p.dtype)
+        idx = [int(i) if int(i) >= 0 else self._nqubits + int(i) for i in x + y + z]
+        if shots is None:
+            return qu.correlation_from_counts(idx, p)
+        u = qu._uniforms((1, int(shots)), status, random_generator)
+        ch = engine.sample_prob_indices(p.detach().reshape(1, -1), self._nqubits, u)[0]
+        return qu.correlation_from_samples(idx, ch, self._nqubits)
+
+    sexpps = sample_expectation_ps
 
     def expectation_ps(self, x: Optional[Sequence[int]] = None, y: Optional[Sequence[int]] = None,
                        z: Optional[Sequence[int]] = None, ps: Optional[Sequence[int]] = None,

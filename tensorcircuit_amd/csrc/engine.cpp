@@ -899,6 +899,124 @@ int tcm_expect_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
   return 0;
 }
 
+// ---- measurement ----------------------------------------------------------
+namespace {
+// chunk prefix sums of the sampler: batch x (2^(n - min(n, 10)) + 1) doubles
+int64_t sample_ws(int n, int64_t batch) {
+  const int cl = n < 10 ? n : 10;
+  return batch * (((int64_t)1 << (n - cl)) + 1) * (int64_t)sizeof(double);
+}
+// blocks of the measurement partial sums and workspace: fixval (u64 per
+// batch item), then part[batch][kMeasBlocks][2]
+constexpr int kMeasBlocks = 1024;
+int64_t measure_ws(int64_t batch) {
+  return batch * (int64_t)sizeof(uint64_t) + batch * kMeasBlocks * 2 * (int64_t)sizeof(double);
+}
+}  // namespace
+
+int tcm_probability(int32_t n, int32_t dtype, const void* state, int64_t batch, void* out, void* stream) {
+  if (n < 1 || n > 40 || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_probability");
+  if (batch == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t total = batch << n;
+  if (dtype == TCM_C64) launch_probability<float>(state, out, total, st);
+  else launch_probability<double>(state, out, total, st);
+  HIP_TRY(hipGetLastError(), "probability launch");
+  return 0;
+}
+
+int tcm_sample_workspace_bytes(int32_t n, int64_t batch, int64_t* bytes) {
+  if (!bytes || n < 1 || n > 40 || batch < 0) return set_error(TCM_E_ARG, "bad arguments to tcm_sample_workspace_bytes");
+  *bytes = sample_ws(n, batch);
+  return 0;
+}
+
+namespace {
+int sample_impl(const char* what, bool prob, int32_t n, int32_t dtype, const void* state, int64_t batch,
+                const double* uniforms, int64_t shots, int64_t* out, void* workspace, int64_t ws_bytes,
+                void* stream) {
+  if (n < 1 || n > 40 || !valid_dtype(dtype) || batch < 0 || shots < 0 ||
+      (batch > 0 && shots > 0 && (!state || !uniforms || !out || !workspace)))
+    return set_error(TCM_E_ARG, "bad arguments to %s", what);
+  if (batch == 0 || shots == 0) return 0;
+  if (batch > 65535 || (shots + 3) / 4 > 0x7fffffffll || ((int64_t)1 << (n - (n < 10 ? n : 10))) > 0x7fffffffll)
+    return set_error(TCM_E_ARG, "%s: batch, shots or state too large for one launch", what);
+  if (ws_bytes < sample_ws(n, batch)) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TCM_C64) launch_sample<float>(state, prob, n, batch, uniforms, shots, out, (double*)workspace, st);
+  else launch_sample<double>(state, prob, n, batch, uniforms, shots, out, (double*)workspace, st);
+  HIP_TRY(hipGetLastError(), "sample launch");
+  return 0;
+}
+}  // namespace
+
+int tcm_sample(int32_t n, int32_t dtype, const void* state, int64_t batch, const double* uniforms, int64_t shots,
+               int64_t* out, void* workspace, int64_t ws_bytes, void* stream) {
+  return sample_impl("tcm_sample", false, n, dtype, state, batch, uniforms, shots, out, workspace, ws_bytes, stream);
+}
+
+int tcm_sample_prob(int32_t n, int32_t dtype, const void* p, int64_t batch, const double* uniforms, int64_t shots,
+                    int64_t* out, void* workspace, int64_t ws_bytes, void* stream) {
+  return sample_impl("tcm_sample_prob", true, n, dtype, p, batch, uniforms, shots, out, workspace, ws_bytes, stream);
+}
+
+int tcm_measure_workspace_bytes(int32_t n, int64_t batch, int64_t* bytes) {
+  if (!bytes || n < 1 || n > 40 || batch < 0) return set_error(TCM_E_ARG, "bad arguments to tcm_measure_workspace_bytes");
+  *bytes = measure_ws(batch);
+  return 0;
+}
+
+int tcm_measure(int32_t n, int32_t dtype, const void* state, int64_t batch, const int32_t* qubits, int32_t nmeas,
+                const double* status, double* outcomes, double* prob, void* workspace, int64_t ws_bytes,
+                void* stream) {
+  if (n < 1 || n > 40 || !valid_dtype(dtype) || batch < 0 || nmeas < 0 || nmeas > n ||
+      (nmeas > 0 && !qubits) || (batch > 0 && nmeas > 0 && (!state || !status || !outcomes || !prob || !workspace)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_measure");
+  uint64_t seen = 0;
+  for (int k = 0; k < nmeas; ++k) {
+    if (qubits[k] < 0 || qubits[k] >= n) return set_error(TCM_E_ARG, "tcm_measure: qubit %d out of range", qubits[k]);
+    const uint64_t bitm = 1ull << (n - 1 - qubits[k]);
+    if (seen & bitm) return set_error(TCM_E_ARG, "tcm_measure: qubit %d repeated", qubits[k]);
+    seen |= bitm;
+  }
+  if (batch == 0 || nmeas == 0) return 0;
+  if (batch > 65535) return set_error(TCM_E_ARG, "tcm_measure: batch too large for one launch");
+  if (ws_bytes < measure_ws(batch)) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  uint64_t* fixval = (uint64_t*)workspace;
+  double* part = (double*)((char*)workspace + batch * (int64_t)sizeof(uint64_t));
+  uint64_t fixed = 0;
+  for (int k = 0; k < nmeas; ++k) {
+    MeasureStep M{};
+    M.bit = n - 1 - qubits[k];
+    M.first = k == 0 ? 1 : 0;
+    // runs of consecutive free index bits, counter bits in ascending order
+    int src = 0;
+    for (int b = 0; b < n;) {
+      if (fixed >> b & 1) {
+        ++b;
+        continue;
+      }
+      int len = 0;
+      while (b + len < n && !(fixed >> (b + len) & 1)) ++len;
+      M.run[M.nrun++] = src | len << 8 | b << 16;
+      src += len;
+      b += len;
+    }
+    M.nfree = src;
+    const int64_t work = (int64_t)1 << M.nfree;
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kMeasBlocks, work / 2048));
+    if (dtype == TCM_C64)
+      launch_measure_step<float>(state, n, batch, M, k, nmeas, status, fixval, prob, outcomes, part, nblk, st);
+    else
+      launch_measure_step<double>(state, n, batch, M, k, nmeas, status, fixval, prob, outcomes, part, nblk, st);
+    fixed |= 1ull << M.bit;
+  }
+  HIP_TRY(hipGetLastError(), "measure launch");
+  return 0;
+}
+
 int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state, void* out_state, int64_t batch,
                         const double* alpha, void* stream) {
   if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out_state)) || state == out_state)

@@ -140,6 +140,24 @@ void launch_inner(const void* a, const void* c, int n, int64_t batch, int nblk, 
                   hipStream_t st);
 void launch_sum_slots(const double* part, int64_t bstride, int64_t nslots, int ncomp, double* out, int64_t batch,
                       hipStream_t st);
+// measurement
+struct MeasureStep {
+  int32_t nfree;     // free (not yet measured) index bits, enumerated
+  int32_t nrun;      // runs of consecutive free bits
+  int32_t bit;       // index bit of the qubit measured in this step
+  int32_t first;     // 1 on the first step: no bits fixed yet
+  int32_t run[34];   // counter bits src .. src+len-1 -> index bits dst ..
+};
+template <typename T>
+void launch_probability(const void* psi, void* out, int64_t total, hipStream_t st);
+// src: states [batch][2^n] (prob false) or probability vectors (prob true)
+template <typename T>
+void launch_sample(const void* src, bool prob, int n, int64_t batch, const double* u, int64_t shots, int64_t* out,
+                   double* cum, hipStream_t st);
+template <typename T>
+void launch_measure_step(const void* psi, int n, int64_t batch, const MeasureStep& M, int k, int nmeas,
+                         const double* status, uint64_t* fixval, double* prob, double* outcome, double* part,
+                         int nblk, hipStream_t st);
 void launch_reduce_gop(const double* part, int64_t pstride, int64_t nslots, const int32_t* gop_list, int gop_single,
                        int ngop_pass, double* gsum, int ngop, int64_t batch, hipStream_t st);
 void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, const int32_t* p_gops,

@@ -1586,6 +1586,210 @@ void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, co
                      gfac, mult, grad, nparams, batch);
 }
 
+// ---------------------------------------------------------------------------
+// Measurement: probability vector (basecircuit.py:510-524), draws from
+// |psi|^2 by inverse CDF (backend.probability_sample,
+// abstract_backend.py:1124-1157) and qubit-by-qubit conditional measurement
+// (measure_jit, basecircuit.py:371-445).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_probability(const V2<T>* __restrict__ psi, T* __restrict__ out, int64_t total) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const V2<T> x = psi[i];
+    out[i] = x.x * x.x + x.y * x.y;
+  }
+}
+
+// weight of index i for the sampler: |psi_i|^2 of a state (PROB false) or
+// p_i of a probability vector (PROB true)
+template <typename T, bool PROB>
+__device__ __forceinline__ double sample_weight(const void* src, int64_t i) {
+  if constexpr (PROB) {
+    return (double)static_cast<const T*>(src)[i];
+  } else {
+    const V2<T> x = static_cast<const V2<T>*>(src)[i];
+    return (double)x.x * x.x + (double)x.y * x.y;
+  }
+}
+
+// cum[b][c + 1] = fp64 sum of the weights over chunk c (2^chunk_log
+// indices); grid (nchunks, batch)
+template <typename T, bool PROB>
+__global__ void __launch_bounds__(256) k_prob_chunks(const void* __restrict__ src, int64_t sstride, int chunk_log,
+                                                     int64_t nchunks, double* __restrict__ cum) {
+  const int64_t c = blockIdx.x, b = blockIdx.y;
+  const int64_t off = b * sstride + (c << chunk_log);
+  const int64_t len = 1ll << chunk_log;
+  double a = 0.0, d = 0.0;
+  for (int64_t i = threadIdx.x; i < len; i += blockDim.x) a += sample_weight<T, PROB>(src, off + i);
+  block_sum2(a, d);
+  if (threadIdx.x == 0) cum[b * (nchunks + 1) + c + 1] = a;
+}
+
+// in-place inclusive prefix sums of cum[b][1..nchunks], cum[b][0] = 0; one
+// 1024-thread block per batch item, each thread owning a contiguous range
+__global__ void __launch_bounds__(1024) k_scan_chunks(double* __restrict__ cum, int64_t nchunks) {
+  __shared__ double sh[1024];
+  double* c = cum + blockIdx.x * (nchunks + 1);
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int64_t per = (nchunks + nt - 1) / nt;
+  const int64_t lo = min(nchunks, (int64_t)t * per), hi = min(nchunks, lo + per);
+  double s = 0.0;
+  for (int64_t i = lo; i < hi; ++i) s += c[1 + i];
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    const double v = t >= off ? sh[t - off] : 0.0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  double run = sh[t] - s;
+  for (int64_t i = lo; i < hi; ++i) {
+    run += c[1 + i];
+    c[1 + i] = run;
+  }
+  if (t == 0) c[0] = 0.0;
+}
+
+// one wave per draw: r = total * (1 - u); the drawn index is the smallest i
+// whose inclusive prefix sum of |psi|^2 reaches r (searchsorted, side left):
+// binary search over the chunk prefixes, then a 64-wide running scan of the
+// chunk.  grid (ceil(shots / 4), batch), 256 threads
+template <typename T, bool PROB>
+__global__ void __launch_bounds__(256) k_sample_draw(const void* __restrict__ src, int64_t sstride, int chunk_log,
+                                                     int64_t nchunks, const double* __restrict__ cum,
+                                                     const double* __restrict__ u, int64_t shots,
+                                                     int64_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t b = blockIdx.y;
+  if (s >= shots) return;  // uniform per wave
+  const double* c = cum + b * (nchunks + 1);
+  const double r = c[nchunks] * (1.0 - u[b * shots + s]);
+  int64_t lo = 0, hi = nchunks - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (c[mid + 1] >= r) hi = mid;
+    else lo = mid + 1;
+  }
+  const int64_t len = 1ll << chunk_log;
+  const int64_t first = lo << chunk_log;
+  const int64_t base = b * sstride + first;
+  double run = c[lo];
+  int64_t found = -1, lastnz = -1;
+  for (int64_t j = 0; j < len; j += 64) {
+    const double p = (j + lane < len) ? sample_weight<T, PROB>(src, base + j + lane) : 0.0;
+    double incl = p;
+    for (int off = 1; off < 64; off <<= 1) {
+      const double v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const unsigned long long valid = (len - j >= 64) ? ~0ull : ((1ull << (len - j)) - 1);
+    const unsigned long long hit = __ballot(run + incl >= r) & valid;
+    const unsigned long long nz = __ballot(p > 0.0) & valid;
+    if (nz) lastnz = j + 63 - __clzll((long long)nz);
+    if (hit) {
+      found = j + __ffsll((long long)hit) - 1;
+      break;
+    }
+    run += __shfl(incl, 63, 64);
+  }
+  // rounding left r just above the chunk's running total: the last index
+  // of the chunk with nonzero probability
+  if (found < 0) found = lastnz >= 0 ? lastnz : 0;
+  if (lane == 0) out[b * shots + s] = first + found;
+}
+
+// partial sums of |psi_i|^2 over the indices that agree with the outcomes
+// fixed so far (fixval[b] on the fixed bits), split by index bit M.bit; the
+// free bits are enumerated by depositing a counter into M.run.
+// grid (nblk, batch); part[b][blk][2]
+template <typename T>
+__global__ void __launch_bounds__(256) k_measure_partial(const V2<T>* __restrict__ psi, int64_t sstride,
+                                                         MeasureStep M, const uint64_t* __restrict__ fixval,
+                                                         double* __restrict__ part) {
+  const int64_t b = blockIdx.y;
+  const V2<T>* s = psi + b * sstride;
+  const uint64_t fv = M.first ? 0ull : fixval[b];
+  const int64_t total = 1ll << M.nfree;
+  double a0 = 0.0, a1 = 0.0;
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < total; m += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t i = fv;
+    for (int r = 0; r < M.nrun; ++r) {
+      const int src = M.run[r] & 0xff, len = (M.run[r] >> 8) & 0xff, dst = M.run[r] >> 16;
+      i |= (((uint64_t)m >> src) & ((1ull << len) - 1)) << dst;
+    }
+    const V2<T> x = s[i];
+    const double p = (double)x.x * x.x + (double)x.y * x.y;
+    if ((i >> M.bit) & 1) a1 += p;
+    else a0 += p;
+  }
+  block_sum2(a0, a1);
+  if (threadIdx.x == 0) {
+    part[(b * gridDim.x + blockIdx.x) * 2] = a0;
+    part[(b * gridDim.x + blockIdx.x) * 2 + 1] = a1;
+  }
+}
+
+// measure_jit's decision for step k (basecircuit.py:419-431): pu = p0 / p,
+// outcome 1 iff status - pu + eps > 0, p *= pu or (1 - pu); one block per
+// batch item
+__global__ void __launch_bounds__(256) k_measure_decide(const double* __restrict__ part, int nblk, int k, int nmeas,
+                                                        int bit, const double* __restrict__ status,
+                                                        uint64_t* __restrict__ fixval, double* __restrict__ prob,
+                                                        double* __restrict__ outcome) {
+  const int64_t b = blockIdx.x;
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) a0 += part[(b * nblk + i) * 2];
+  block_sum2(a0, a1);
+  if (threadIdx.x == 0) {
+    const double p = k == 0 ? 1.0 : prob[b];
+    const double pu = a0 / p;
+    const double r = status[b * nmeas + k];
+    const double eps = 0.31415926 * 1e-12;
+    const int o = (r - pu + eps > 0.0) ? 1 : 0;
+    prob[b] = p * (o ? 1.0 - pu : pu);
+    fixval[b] = (k == 0 ? 0ull : fixval[b]) | ((uint64_t)o << bit);
+    outcome[b * nmeas + k] = (double)o;
+  }
+}
+
+template <typename T>
+void launch_probability(const void* psi, void* out, int64_t total, hipStream_t st) {
+  hipLaunchKernelGGL(k_probability<T>, dim3(grid_for(total)), dim3(256), 0, st, (const V2<T>*)psi, (T*)out, total);
+}
+
+template <typename T>
+void launch_sample(const void* src, bool prob, int n, int64_t batch, const double* u, int64_t shots, int64_t* out,
+                   double* cum, hipStream_t st) {
+  const int chunk_log = n < 10 ? n : 10;
+  const int64_t nchunks = 1ll << (n - chunk_log);
+  const int64_t sstride = 1ll << n;
+  const dim3 gc((unsigned)nchunks, (unsigned)batch), gd((unsigned)((shots + 3) / 4), (unsigned)batch);
+  if (prob)
+    hipLaunchKernelGGL((k_prob_chunks<T, true>), gc, dim3(256), 0, st, src, sstride, chunk_log, nchunks, cum);
+  else
+    hipLaunchKernelGGL((k_prob_chunks<T, false>), gc, dim3(256), 0, st, src, sstride, chunk_log, nchunks, cum);
+  hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)batch), dim3(1024), 0, st, cum, nchunks);
+  if (prob)
+    hipLaunchKernelGGL((k_sample_draw<T, true>), gd, dim3(256), 0, st, src, sstride, chunk_log, nchunks,
+                       (const double*)cum, u, shots, out);
+  else
+    hipLaunchKernelGGL((k_sample_draw<T, false>), gd, dim3(256), 0, st, src, sstride, chunk_log, nchunks,
+                       (const double*)cum, u, shots, out);
+}
+
+template <typename T>
+void launch_measure_step(const void* psi, int n, int64_t batch, const MeasureStep& M, int k, int nmeas,
+                         const double* status, uint64_t* fixval, double* prob, double* outcome, double* part,
+                         int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_measure_partial<T>, dim3((unsigned)nblk, (unsigned)batch), dim3(256), 0, st,
+                     (const V2<T>*)psi, (int64_t)1 << n, M, (const uint64_t*)fixval, part);
+  hipLaunchKernelGGL(k_measure_decide, dim3((unsigned)batch), dim3(256), 0, st, (const double*)part, nblk, k, nmeas,
+                     M.bit, status, fixval, prob, outcome);
+}
+
 // explicit instantiations used by engine.cpp
 #define TCM_INST(T)                                                                                            \
   template void launch_prep<T>(const Sched&, const DeviceTables&, const void*, const void*, int64_t, void*,   \
@@ -1611,7 +1815,12 @@ void launch_grad_assemble(const double* gsum, int ngop, const int32_t* p_ptr, co
   template void launch_fused_fwd_multi<T>(const MultiJobs&, int64_t, int, size_t, hipStream_t);                 \
   template void launch_pauli_tile<T>(const DPPass&, const DPPass*, int, const DPGroupT*, const DPTermT*,        \
                                      const uint32_t*, const void*, const void*, void*, int, const double*,       \
-                                     double*, int64_t, int64_t, bool, bool, hipStream_t);
+                                     double*, int64_t, int64_t, bool, bool, hipStream_t);                      \
+  template void launch_probability<T>(const void*, void*, int64_t, hipStream_t);                               \
+  template void launch_sample<T>(const void*, bool, int, int64_t, const double*, int64_t, int64_t*, double*,   \
+                                 hipStream_t);                                                                 \
+  template void launch_measure_step<T>(const void*, int, int64_t, const MeasureStep&, int, int, const double*,  \
+                                       uint64_t*, double*, double*, double*, int, hipStream_t);
 
 TCM_INST(float)
 TCM_INST(double)

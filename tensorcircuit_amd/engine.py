@@ -442,6 +442,85 @@ def inner(a: torch.Tensor, b: torch.Tensor, n: int) -> torch.Tensor:
 
 
 # ---------------------------------------------------------------------------
+# measurement
+# ---------------------------------------------------------------------------
+
+
+def _states(state: torch.Tensor, n: int) -> Tuple[torch.Tensor, int]:
+    dev = device()
+    if state.dtype not in (torch.complex64, torch.complex128):
+        state = state.to(torch.complex64)
+    s = state.to(dev).reshape(-1, 1 << n).contiguous()
+    return s, _CODE["complex64" if s.dtype == torch.complex64 else "complex128"]
+
+
+def probability(state: torch.Tensor, n: int) -> torch.Tensor:
+    """|psi|^2 in the state's real dtype, shape [batch, 2^n] (tcm_probability)."""
+    s, code = _states(state, n)
+    out = torch.empty(s.shape, dtype=_RDT["complex64" if code == _lib.TCM_C64 else "complex128"], device=s.device)
+    check(lib.tcm_probability(n, code, s.data_ptr(), s.shape[0], out.data_ptr(), _stream()), "tcm_probability")
+    return out
+
+
+def sample_indices(state: torch.Tensor, n: int, uniforms: torch.Tensor) -> torch.Tensor:
+    """int64 [batch, shots]: inverse-CDF draws from |psi_b|^2 for the uniforms
+    [batch, shots] in [0, 1] (tcm_sample; the reference's
+    backend.probability_sample, abstract_backend.py:1124-1157)."""
+    s, code = _states(state, n)
+    bt = s.shape[0]
+    u = uniforms.to(device=s.device, dtype=torch.float64).reshape(bt, -1).contiguous()
+    shots = u.shape[1]
+    out = torch.empty(bt, shots, dtype=torch.int64, device=s.device)
+    nb = _i64()
+    check(lib.tcm_sample_workspace_bytes(n, bt, C.byref(nb)), "workspace")
+    ws = _workspace(nb.value, s.device)
+    check(lib.tcm_sample(n, code, s.data_ptr(), bt, u.data_ptr(), shots, out.data_ptr(), ws.data_ptr(), ws.numel(),
+                         _stream()), "tcm_sample")
+    return out
+
+
+def sample_prob_indices(p: torch.Tensor, n: int, uniforms: torch.Tensor) -> torch.Tensor:
+    """int64 [batch, shots]: inverse-CDF draws from probability vectors p
+    [batch, 2^n] (normalised by their sum) for the uniforms [batch, shots]
+    (tcm_sample_prob)."""
+    dev = device()
+    pt = p.to(dev)
+    if pt.dtype not in (torch.float32, torch.float64):
+        pt = pt.to(torch.float32)
+    pt = pt.reshape(-1, 1 << n).contiguous()
+    code = _lib.TCM_C64 if pt.dtype == torch.float32 else _lib.TCM_C128
+    bt = pt.shape[0]
+    u = uniforms.to(device=dev, dtype=torch.float64).reshape(bt, -1).contiguous()
+    shots = u.shape[1]
+    out = torch.empty(bt, shots, dtype=torch.int64, device=dev)
+    nb = _i64()
+    check(lib.tcm_sample_workspace_bytes(n, bt, C.byref(nb)), "workspace")
+    ws = _workspace(nb.value, dev)
+    check(lib.tcm_sample_prob(n, code, pt.data_ptr(), bt, u.data_ptr(), shots, out.data_ptr(), ws.data_ptr(),
+                              ws.numel(), _stream()), "tcm_sample_prob")
+    return out
+
+
+def measure(state: torch.Tensor, n: int, qubits: Sequence[int], status: torch.Tensor
+            ) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Sequential measurement of `qubits` with external randomness status
+    [batch, len(qubits)] (tcm_measure; measure_jit, basecircuit.py:371-445):
+    outcomes fp64 [batch, k] and the probability of that record fp64 [batch]."""
+    s, code = _states(state, n)
+    bt = s.shape[0]
+    q = (C.c_int32 * max(len(qubits), 1))(*[int(x) for x in qubits])
+    st = status.to(device=s.device, dtype=torch.float64).reshape(bt, len(qubits)).contiguous()
+    outc = torch.empty(bt, len(qubits), dtype=torch.float64, device=s.device)
+    prob = torch.ones(bt, dtype=torch.float64, device=s.device)
+    nb = _i64()
+    check(lib.tcm_measure_workspace_bytes(n, bt, C.byref(nb)), "workspace")
+    ws = _workspace(nb.value, s.device)
+    check(lib.tcm_measure(n, code, s.data_ptr(), bt, q, len(qubits), st.data_ptr(), outc.data_ptr(), prob.data_ptr(),
+                          ws.data_ptr(), ws.numel(), _stream()), "tcm_measure")
+    return outc, prob
+
+
+# ---------------------------------------------------------------------------
 # autograd bridges with vmap rules
 # ---------------------------------------------------------------------------
 
@@ -658,3 +737,28 @@ class OpExpFn(torch.autograd.Function):
             gg = g.to(st.dtype)
             out = (torch.conj(gg) * a + gg * b).reshape(st.shape)
         return out, None, None, None
+
+
+class ProbFn(torch.autograd.Function):
+    """psi [..., N] -> |psi|^2 (tcm_probability); backward 2 g psi (the torch
+    convention for real losses of complex inputs)."""
+
+    @staticmethod
+    def forward(st, n):
+        return probability(st, n).reshape(st.shape)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, _ = inputs
+        ctx.save_for_backward(st)
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        return 2.0 * g.to(st.real.dtype) * st, None
+
+    @staticmethod
+    def vmap(info, in_dims, st, n):
+        B = info.batch_size
+        s = _to_batch(st, in_dims[0], B, (1 << n,))
+        return ProbFn.apply(s, n), 0
