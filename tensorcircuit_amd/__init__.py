@@ -8,7 +8,7 @@ fallback (importing the package fails when the library is not built).
 __version__ = "0.1.0"
 
 from . import _lib  # noqa: F401  (fails loudly without libtcm.so)
-from . import cons, experimental, gates, interfaces, quantum, templates, torchnn
+from . import cons, experimental, gates, interfaces, quantum, templates, torchnn, translation
 from .backend import backend
 from .circuit import Circuit, expectation
 from .cons import get_backend, runtime_dtype, set_backend, set_dtype, set_jit

@@ -496,6 +496,54 @@ class Circuit:
                     c, *index, **dict(d.get("parameters", {})))
         return c
 
+    def to_json(self, file: Optional[str] = None, simplified: bool = False) -> Any:
+        """JSON gate list of the circuit (abstractcircuit.py:932-950)."""
+        import json
+
+        from .translation import qir2json
+
+        tcqasm = qir2json(self.to_qir(), simplified=simplified)
+        if file is not None:
+            with open(file, "w") as f:
+                json.dump(tcqasm, f)
+        return json.dumps(tcqasm)
+
+    @classmethod
+    def from_json(cls, jsonstr: Any, circuit_params: Optional[Dict[str, Any]] = None) -> "Circuit":
+        """abstractcircuit.py:1036-1056."""
+        import json
+
+        from .translation import json2qir
+
+        if isinstance(jsonstr, str):
+            jsonstr = json.loads(jsonstr)
+        return cls.from_qir(json2qir(jsonstr), circuit_params)
+
+    @classmethod
+    def from_json_file(cls, file: str, circuit_params: Optional[Dict[str, Any]] = None) -> "Circuit":
+        """abstractcircuit.py:1058-1074."""
+        import json
+
+        with open(file, "r") as f:
+            return cls.from_json(json.load(f), circuit_params)
+
+    @classmethod
+    def from_qsim_file(cls, file: str, circuit_params: Optional[Dict[str, Any]] = None) -> "Circuit":
+        """A circuit in qsim's text format: qubit count, then one gate per
+        line (abstractcircuit.py:952-966)."""
+        with open(file, "r") as f:
+            lines = f.readlines()
+        params = dict(circuit_params or {})
+        if "nqubits" not in params:
+            params["nqubits"] = int(lines[0])
+        return cls._apply_qsim(cls(**params), lines)
+
+    @staticmethod
+    def _apply_qsim(c: "Circuit", qsim_str: List[str]) -> "Circuit":
+        from .translation import apply_qsim
+
+        return apply_qsim(c, qsim_str[1:])
+
     def gate_count(self, gate_list: Optional[Union[str, Sequence[str]]] = None) -> int:
         """abstractcircuit.py gate_count."""
         if gate_list is None:

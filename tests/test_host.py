@@ -271,3 +271,49 @@ def test_compiled_pass_signatures(tmp_path, monkeypatch):
         assert sigs, mode
         for s in sigs:
             assert len(s.split(",")) == nparams, (mode, s)
+
+
+def test_json_round_trip_and_qsim_parse(tmp_path):
+    """to_json / from_json (abstractcircuit.py:932-1074, translation.py:587-702)
+    and qsim text (abstractcircuit.py:953-1034) on the host: same gate
+    names, qubits and parameters; matrices equal the factories'."""
+    import json
+
+    import tensorcircuit_amd as tc
+    from tensorcircuit_amd import translation as tr
+
+    c = tc.Circuit(3)
+    c.h(0)
+    c.CNOT(1, 2)
+    c.rxx(0, 2, theta=0.3)
+    c.crx(0, 1, theta=-0.8)
+    c.r(1, theta=1.0, alpha=0.2)
+    c.toffoli(0, 2, 1)
+    c.ccnot(0, 1, 2)
+    c.any(0, 1, unitary=np.kron(sv.gate_matrix("h"), sv.gate_matrix("rx", theta=0.4)))
+    for simplified in (False, True):
+        s = c.to_json(simplified=simplified)
+        d = json.loads(s)
+        assert [x["name"] for x in d] == ["h", "cnot", "rxx", "crx", "r", "toffoli", "toffoli", "any"]
+        c2 = tc.Circuit.from_json(s)
+        q1, q2 = c.to_qir(), c2.to_qir()
+        assert [tuple(x["index"]) for x in q1] == [tuple(x["index"]) for x in q2]
+        for a, b in zip(q1, q2):
+            np.testing.assert_allclose(tr._matrix(a), tr._matrix(b), atol=1e-12)
+    f = tmp_path / "c.json"
+    c.to_json(file=str(f))
+    assert len(tc.Circuit.from_json_file(str(f)).to_qir()) == 8
+    # U parameters reproduce a 2x2 unitary up to a global phase
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        u = sv.haar_su2(rng) * np.exp(1j * rng.random())
+        v = tc.gates.u_matrix(*tr.get_u_parameter(u))
+        ph = u[0, 0] / v[0, 0] if abs(v[0, 0]) > 1e-6 else u[0, 1] / v[0, 1]
+        np.testing.assert_allclose(u, ph * v, atol=1e-10)
+    # qsim text: "time name qubits [params]"
+    q = tmp_path / "c.qsim"
+    q.write_text("3\n0 h 0\n0 x_1_2 1\n1 cz 0 1\n1 rz 2 0.25\n2 fs 1 2 0.5 0.3\n3 hz_1_2 2\n3 t 0\n")
+    c3 = tc.Circuit.from_qsim_file(str(q))
+    names = [x["name"] for x in c3.to_qir()]
+    assert names == ["h", "rx", "cz", "rz", "iswap", "cphase", "wroot", "phase"]
+    np.testing.assert_allclose(tr._matrix(c3.to_qir()[4]), tc.gates.iswap_matrix(-0.5), atol=1e-12)
