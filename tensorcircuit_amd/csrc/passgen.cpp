@@ -418,7 +418,8 @@ int adj_partials(int dt, const DPass& dp) {
   const int NT = 1 << dp.tthreads_log;
   const size_t tile = (size_t)(dt == 0 ? 8 : 16) << dp.k;
   const size_t g = (size_t)std::max(1, dp.ngop);
-  auto blocks = [&](int per) { return (160u * 1024u) / (tile + 8 * g * (size_t)per); };
+  const size_t rb = dt == 0 ? 4 : 8;  // bytes per partial (working precision)
+  auto blocks = [&](int per) { return (160u * 1024u) / (tile + rb * g * (size_t)per); };
   return blocks(NT / 16) == blocks(NT / 64) ? NT / 16 : NT / 64;
 }
 
@@ -508,8 +509,36 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
         if (op.kind == OK_DG && op.nmem > 0) {
           G.line("{");
           for (int j = 0; j < nreg; ++j) G.line(fmt("const T im%d = imcj(%s, %s);", j, G.lv(j).c_str(), G.pv(j).c_str()));
+          // Z-generator sums sum_j (-1)^{bit b of j} im_j for every needed
+          // bit b share one butterfly: level b splits the running pair sums
+          // into sums (next level) and differences (summed into zs<b>)
+          int zneed = 0;
+          for (int q = 0; q < op.nmem; ++q)
+            if (S.dmem[op.mem0 + q].gen == GN_Z) zneed |= 1 << S.dmem[op.mem0 + q].s0;
+          if (zneed) {
+            std::vector<std::string> v;
+            for (int j = 0; j < nreg; ++j) v.push_back(fmt("im%d", j));
+            for (int bit = 0; bit < R && (zneed >> bit); ++bit) {
+              std::vector<std::string> nx;
+              std::string zs;
+              for (size_t j = 0; j + 1 < v.size(); j += 2) {
+                if (zneed >> (bit + 1)) {
+                  nx.push_back(fmt("bs%d_%d", bit, (int)(j / 2)));
+                  G.line(fmt("const T %s = %s + %s;", nx.back().c_str(), v[j].c_str(), v[j + 1].c_str()));
+                }
+                if (zneed >> bit & 1) zs += fmt("%s(%s - %s)", zs.empty() ? "" : " + ", v[j].c_str(), v[j + 1].c_str());
+              }
+              if (zneed >> bit & 1) G.line(fmt("const T zs%d = %s;", bit, zs.c_str()));
+              v.swap(nx);
+            }
+          }
           for (int q = 0; q < op.nmem; ++q) {
             const DMember& M = S.dmem[op.mem0 + q];
+            if (M.gen == GN_Z) {
+              G.line(fmt("{ const T gs = %s(zs%d);", rsum, M.s0));
+              G.line(fmt("  if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs; }", lmask, lmask, M.gloc * PR, lshift));
+              continue;
+            }
             std::string e;
             for (int j = 0; j < nreg; ++j) {
               int sg = 0;
@@ -519,14 +548,14 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
               if (sg) e += fmt(" %s im%d", sg > 0 ? "+" : "-", j);
             }
             if (e.empty()) e = " + 0";
-            G.line(fmt("{ const T ga = (T)0%s; const double gs = (double)%s(ga);", e.c_str(), rsum));
+            G.line(fmt("{ const T ga = (T)0%s; const T gs = %s(ga);", e.c_str(), rsum));
             G.line(fmt("  if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs; }", lmask, lmask, M.gloc * PR, lshift));
           }
           G.line("}");
         } else if (op.gen != GN_NONE) {
           G.line("{");
           if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return false;
-          G.line(fmt("const double gs = (double)%s(ga);", rsum));
+          G.line(fmt("const T gs = %s(ga);", rsum));
           G.line(fmt("if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs;", lmask, lmask, op.gloc * PR, lshift));
           G.line("}");
         }
@@ -539,8 +568,9 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     // through it, then lambda (four barriers instead of one), for half the
     // LDS of two tiles and twice the resident blocks per CU.
     G.line(fmt("__shared__ cT tb[%d];", 1 << k));
-    // gradient partials: one fp64 per wave and trainable gate
-    G.line(fmt("__shared__ double red[%d];", std::max(1, dp.ngop) * PR));
+    // gradient partials: one per 16-lane row (or wave) and trainable gate, in
+    // the working precision; the block sum below is fp64
+    G.line(fmt("__shared__ T red[%d];", std::max(1, dp.ngop) * PR));
     {
       std::string decl = "cT";
       for (int j = 0; j < nreg; ++j) decl += fmt("%s p%d, l%d", j ? "," : "", j, j);
@@ -622,7 +652,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     if (dp.ngop > 0) {
       G.line(fmt("for (unsigned q = tid; q < %du; q += %du) {", dp.ngop, NT));
       G.line("  double sum = 0.0;");
-      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += red[q * %d + w];", PR, PR));
+      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += (double)red[q * %d + w];", PR, PR));
       G.line(fmt("  part[b * pstride + (long long)q * %lldll + (long long)blk] = sum;", 1ll << nbl));
       G.line("}");
     }
@@ -691,7 +721,8 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
       // (160 KiB per CU), so registers never cut occupancy below that
       const size_t cbytes = dt == 0 ? 8 : 16;
       const size_t lds = (cbytes << dp.k) +
-                         (mode == 0 ? 0 : 8 * (size_t)std::max(1, dp.ngop) * (size_t)adj_partials(dt, dp));
+                         (mode == 0 ? 0 : (dt == 0 ? 4 : 8) * (size_t)std::max(1, dp.ngop) *
+                                              (size_t)adj_partials(dt, dp));
       const int blocks_cu = (int)std::max<size_t>(1, (160u * 1024u) / lds);
       const int waves = std::max(1, std::min(8, blocks_cu * (NT / 64) / 4));
       // parameter lists: jit_launch_fwd / jit_launch_adj pass exactly these

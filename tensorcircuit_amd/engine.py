@@ -762,3 +762,38 @@ class ProbFn(torch.autograd.Function):
         B = info.batch_size
         s = _to_batch(st, in_dims[0], B, (1 << n,))
         return ProbFn.apply(s, n), 0
+
+
+# ---------------------------------------------------------------------------
+# HIP graph capture of a whole step
+# ---------------------------------------------------------------------------
+
+
+class CapturedStep:
+    """A step -- a callable that only enqueues device work on the current
+    stream through this engine (forward / forward_multi / expectations /
+    energy_grad) -- captured once into a HIP graph and replayed with one
+    launch.  Inputs are the tensors the callable closes over: update them in
+    place between replays; ``replay()`` returns the tensors the captured call
+    returned, overwritten by each replay.
+
+    The callable runs ``warmup`` times eagerly first (on a side stream of
+    default priority) so that device tables, compiled kernels and workspace
+    sizes exist before capture; the entry points record no memset nodes."""
+
+    def __init__(self, fn, warmup: int = 2):
+        device()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(max(1, warmup)):
+                fn()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = fn()
+
+    def replay(self):
+        self.graph.replay()
+        return self.out
