@@ -191,6 +191,26 @@ int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
                         void* out_state, int64_t batch, const double* alpha,
                         void* stream);
 
+/* Sparse form of the sum: *nnz = nterms * 2^nqubits entries, written by
+ * tcm_pauli_coo as rows / cols (device int64 [nnz]) and values (device
+ * complex [nnz] of dtype): term t's entry t * 2^n + y sits at (y, y ^ flip)
+ * with value w_t (-i)^ny (-1)^popcount(y & yzmask) -- the reference's
+ * PauliString2COO / ps2coo_core (quantum.py:1415-1485); duplicate
+ * coordinates of different terms add (PauliStringSum2COO,
+ * quantum.py:1304-1356). */
+int tcm_pauli_nnz(const tcm_pauli* ham, int64_t* nnz);
+int tcm_pauli_coo(const tcm_pauli* ham, int32_t dtype, int64_t* rows,
+                  int64_t* cols, void* vals, void* stream);
+
+/* out = <psi|A|psi> (complex fp64 (re, im)) for a 2^n x 2^n CSR matrix A
+ * (rowptr int64 [2^n + 1], cols int64, vals complex of dtype, all device),
+ * as sparse_expectation (templates/measurements.py:177-191). */
+int tcm_csr_expect_workspace_bytes(int32_t nqubits, int64_t* bytes);
+int tcm_csr_expect(int32_t nqubits, int32_t dtype, const int64_t* rowptr,
+                   const int64_t* cols, const void* vals, const void* state,
+                   double* out, void* workspace, int64_t ws_bytes,
+                   void* stream);
+
 /* out[b] = <a_b|b_b> (complex fp64) -- expectation of general operators after
  * they were applied to a copy of the state (circuit.py:914-990). */
 int tcm_inner_workspace_bytes(int32_t nqubits, int64_t batch, int64_t* bytes);

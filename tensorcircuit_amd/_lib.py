@@ -98,6 +98,10 @@ SIGNATURES = {
     "tcm_inner_workspace_bytes": (C.c_int, [_i32, _i64, C.POINTER(_i64)]),
     "tcm_inner": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tcm_probability": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _vp]),
+    "tcm_pauli_nnz": (C.c_int, [_vp, C.POINTER(_i64)]),
+    "tcm_pauli_coo": (C.c_int, [_vp, _i32, _vp, _vp, _vp, _vp]),
+    "tcm_csr_expect_workspace_bytes": (C.c_int, [_i32, C.POINTER(_i64)]),
+    "tcm_csr_expect": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "tcm_sample_workspace_bytes": (C.c_int, [_i32, _i64, C.POINTER(_i64)]),
     "tcm_sample": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tcm_sample_prob": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),

@@ -20,6 +20,7 @@ struct tcm_program {
 
 struct PauliDev {
   void* terms = nullptr;
+  void* groups = nullptr;  // DPGroup per flip mask (sparse form only)
   void* tpasses[2] = {nullptr, nullptr};
   void* tgroups[2] = {nullptr, nullptr};
   void* tterms[2] = {nullptr, nullptr};
@@ -260,6 +261,9 @@ int ensure_pauli(tcm_pauli* H, int dt, PauliDev** out) {
   PauliDev& pd = H->dev[dev];
   if (!pd.terms) {
     if (int rc = upload(H->terms, &pd.terms)) return rc;
+  }
+  if (!pd.groups) {
+    if (int rc = upload(H->groups, &pd.groups)) return rc;
   }
   const PauliTileSched& S = H->ts[dt];
   if (S.ok && !pd.tpasses[dt]) {
@@ -856,6 +860,7 @@ int tcm_pauli_free(tcm_pauli* ham) {
     PauliDev& pd = ham->dev[d];
     (void)hipSetDevice((int)d);
     if (pd.terms) (void)hipFree(pd.terms);
+    if (pd.groups) (void)hipFree(pd.groups);
     for (int dt = 0; dt < 2; ++dt) {
       void* ps[] = {pd.tpasses[dt], pd.tgroups[dt], pd.tterms[dt], pd.truns[dt]};
       for (void* q : ps)
@@ -1014,6 +1019,58 @@ int tcm_measure(int32_t n, int32_t dtype, const void* state, int64_t batch, cons
     fixed |= 1ull << M.bit;
   }
   HIP_TRY(hipGetLastError(), "measure launch");
+  return 0;
+}
+
+int tcm_pauli_nnz(const tcm_pauli* ham, int64_t* nnz) {
+  if (!ham || !nnz) return set_error(TCM_E_ARG, "bad arguments to tcm_pauli_nnz");
+  *nnz = (int64_t)ham->terms.size() << ham->n;
+  return 0;
+}
+
+int tcm_pauli_coo(const tcm_pauli* ham, int32_t dtype, int64_t* rows, int64_t* cols, void* vals, void* stream) {
+  if (!ham || !valid_dtype(dtype) || !rows || !cols || !vals)
+    return set_error(TCM_E_ARG, "bad arguments to tcm_pauli_coo");
+  if (ham->terms.empty()) return 0;
+  if (ham->n > 34) return set_error(TCM_E_ARG, "tcm_pauli_coo: 2^%d rows per term is too many entries", ham->n);
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  PauliDev* pd = nullptr;
+  if (int rc = ensure_pauli(H, dtype, &pd)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nt = (int64_t)H->terms.size();
+  if (dtype == TCM_C64)
+    launch_pauli_coo<float>((const DPGroup*)pd->groups, (int)H->groups.size(), (const DPTerm*)pd->terms, H->n, nt,
+                            rows, cols, vals, st);
+  else
+    launch_pauli_coo<double>((const DPGroup*)pd->groups, (int)H->groups.size(), (const DPTerm*)pd->terms, H->n, nt,
+                             rows, cols, vals, st);
+  HIP_TRY(hipGetLastError(), "pauli_coo launch");
+  return 0;
+}
+
+namespace {
+constexpr int kCsrBlocks = 1024;
+}
+
+int tcm_csr_expect_workspace_bytes(int32_t n, int64_t* bytes) {
+  if (!bytes || n < 1 || n > 40) return set_error(TCM_E_ARG, "bad arguments to tcm_csr_expect_workspace_bytes");
+  *bytes = 2 * kCsrBlocks * (int64_t)sizeof(double);
+  return 0;
+}
+
+int tcm_csr_expect(int32_t n, int32_t dtype, const int64_t* rowptr, const int64_t* cols, const void* vals,
+                   const void* state, double* out, void* workspace, int64_t ws_bytes, void* stream) {
+  if (n < 1 || n > 40 || !valid_dtype(dtype) || !rowptr || !cols || !vals || !state || !out || !workspace)
+    return set_error(TCM_E_ARG, "bad arguments to tcm_csr_expect");
+  if (ws_bytes < 2 * kCsrBlocks * (int64_t)sizeof(double)) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nrows = (int64_t)1 << n;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kCsrBlocks, nrows / 256));
+  if (dtype == TCM_C64)
+    launch_csr_expect<float>(rowptr, cols, vals, state, nrows, nblk, (double*)workspace, out, st);
+  else
+    launch_csr_expect<double>(rowptr, cols, vals, state, nrows, nblk, (double*)workspace, out, st);
+  HIP_TRY(hipGetLastError(), "csr_expect launch");
   return 0;
 }
 

@@ -150,6 +150,12 @@ struct MeasureStep {
 };
 template <typename T>
 void launch_probability(const void* psi, void* out, int64_t total, hipStream_t st);
+template <typename T>
+void launch_pauli_coo(const DPGroup* groups, int ngroups, const DPTerm* terms, int n, int64_t nterms, int64_t* rows,
+                      int64_t* cols, void* vals, hipStream_t st);
+template <typename T>
+void launch_csr_expect(const int64_t* rowptr, const int64_t* cols, const void* vals, const void* psi, int64_t nrows,
+                       int nblk, double* part, double* out, hipStream_t st);
 // src: states [batch][2^n] (prob false) or probability vectors (prob true)
 template <typename T>
 void launch_sample(const void* src, bool prob, int n, int64_t batch, const double* u, int64_t shots, int64_t* out,

@@ -1790,6 +1790,80 @@ void launch_measure_step(const void* psi, int n, int64_t batch, const MeasureSte
                      M.bit, status, fixval, prob, outcome);
 }
 
+// ---------------------------------------------------------------------------
+// Sparse form of a Pauli sum (PauliStringSum2COO / ps2coo_core,
+// quantum.py:1304-1485) and <psi|H|psi> over a CSR matrix
+// (sparse_expectation, templates/measurements.py:177-191).
+// ---------------------------------------------------------------------------
+// entry t * N + y of term t (group g): row y, column y ^ flip,
+// value w (-i)^ny (-1)^popc(y & yz); grid-stride over nterms * N
+template <typename T>
+__global__ void k_pauli_coo(const DPGroup* __restrict__ groups, int ngroups, const DPTerm* __restrict__ terms,
+                            int n, int64_t* __restrict__ rows, int64_t* __restrict__ cols, V2<T>* __restrict__ vals,
+                            int64_t total) {
+  const int64_t N = 1ll << n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(e >> n);
+    const uint64_t y = (uint64_t)(e & (N - 1));
+    int g = 0;
+    while (g + 1 < ngroups && groups[g + 1].t0 <= t) ++g;
+    const DPTerm tm = terms[t];
+    const double sg = (__popcll(y & tm.yz) & 1) ? -tm.w : tm.w;
+    double re = 0.0, im = 0.0;
+    switch (tm.ny & 3) {  // (-i)^ny
+      case 0: re = sg; break;
+      case 1: im = -sg; break;
+      case 2: re = -sg; break;
+      default: im = sg; break;
+    }
+    rows[e] = (int64_t)y;
+    cols[e] = (int64_t)(y ^ groups[g].flip);
+    vals[e] = mk<T>((T)re, (T)im);
+  }
+}
+
+// part[blk] = sum over rows r of this block's grid-stride share of
+// conj(psi_r) * sum_k vals_k psi_{cols_k} (fp64); one thread per row
+template <typename T>
+__global__ void __launch_bounds__(256) k_csr_expect(const int64_t* __restrict__ rowptr, const int64_t* __restrict__ cols,
+                                                    const V2<T>* __restrict__ vals, const V2<T>* __restrict__ psi,
+                                                    int64_t nrows, double* __restrict__ part) {
+  double er = 0.0, ei = 0.0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * blockDim.x) {
+    double hr = 0.0, hi = 0.0;
+    for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+      const V2<T> v = vals[k], x = psi[cols[k]];
+      hr += (double)v.x * x.x - (double)v.y * x.y;
+      hi += (double)v.x * x.y + (double)v.y * x.x;
+    }
+    const V2<T> p = psi[r];
+    er += (double)p.x * hr + (double)p.y * hi;
+    ei += (double)p.x * hi - (double)p.y * hr;
+  }
+  block_sum2(er, ei);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = er;
+    part[2 * blockIdx.x + 1] = ei;
+  }
+}
+
+template <typename T>
+void launch_pauli_coo(const DPGroup* groups, int ngroups, const DPTerm* terms, int n, int64_t nterms, int64_t* rows,
+                      int64_t* cols, void* vals, hipStream_t st) {
+  const int64_t total = nterms << n;
+  hipLaunchKernelGGL(k_pauli_coo<T>, dim3(grid_for(total)), dim3(256), 0, st, groups, ngroups, terms, n, rows, cols,
+                     (V2<T>*)vals, total);
+}
+
+template <typename T>
+void launch_csr_expect(const int64_t* rowptr, const int64_t* cols, const void* vals, const void* psi, int64_t nrows,
+                       int nblk, double* part, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_csr_expect<T>, dim3((unsigned)nblk), dim3(256), 0, st, rowptr, cols, (const V2<T>*)vals,
+                     (const V2<T>*)psi, nrows, part);
+  hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(256), 0, st, (const double*)part, (int64_t)2 * nblk, (int64_t)nblk, 2,
+                     out);
+}
+
 // explicit instantiations used by engine.cpp
 #define TCM_INST(T)                                                                                            \
   template void launch_prep<T>(const Sched&, const DeviceTables&, const void*, const void*, int64_t, void*,   \
@@ -1820,7 +1894,11 @@ void launch_measure_step(const void* psi, int n, int64_t batch, const MeasureSte
   template void launch_sample<T>(const void*, bool, int, int64_t, const double*, int64_t, int64_t*, double*,   \
                                  hipStream_t);                                                                 \
   template void launch_measure_step<T>(const void*, int, int64_t, const MeasureStep&, int, int, const double*,  \
-                                       uint64_t*, double*, double*, double*, int, hipStream_t);
+                                       uint64_t*, double*, double*, double*, int, hipStream_t);                \
+  template void launch_pauli_coo<T>(const DPGroup*, int, const DPTerm*, int, int64_t, int64_t*, int64_t*, void*,  \
+                                    hipStream_t);                                                              \
+  template void launch_csr_expect<T>(const int64_t*, const int64_t*, const void*, const void*, int64_t, int,      \
+                                     double*, double*, hipStream_t);
 
 TCM_INST(float)
 TCM_INST(double)

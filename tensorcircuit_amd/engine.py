@@ -479,6 +479,41 @@ def sample_indices(state: torch.Tensor, n: int, uniforms: torch.Tensor) -> torch
     return out
 
 
+def pauli_coo(ham: "PauliSum", dtype: str = "complex64") -> torch.Tensor:
+    """The Pauli sum as a coalesced 2^n x 2^n torch sparse COO matrix on the
+    device; entries assembled by tcm_pauli_coo (ps2coo_core,
+    quantum.py:1461-1485), duplicates summed (PauliStringSum2COO)."""
+    dev = device()
+    nnz = _i64()
+    check(lib.tcm_pauli_nnz(ham.handle, C.byref(nnz)), "tcm_pauli_nnz")
+    N = 1 << ham.n
+    rows = torch.empty(nnz.value, dtype=torch.int64, device=dev)
+    cols = torch.empty(nnz.value, dtype=torch.int64, device=dev)
+    vals = torch.empty(nnz.value, dtype=_CDT[dtype], device=dev)
+    if nnz.value:
+        check(lib.tcm_pauli_coo(ham.handle, _CODE[dtype], rows.data_ptr(), cols.data_ptr(), vals.data_ptr(),
+                                _stream()), "tcm_pauli_coo")
+    return torch.sparse_coo_tensor(torch.stack([rows, cols]), vals, (N, N)).coalesce()
+
+
+def csr_expectation(h: torch.Tensor, state: torch.Tensor, n: int) -> torch.Tensor:
+    """complex <psi|H|psi> for a sparse (COO or CSR) H by tcm_csr_expect."""
+    s, code = _states(state, n)
+    hc = h.to(device=s.device, dtype=s.dtype)
+    if hc.layout != torch.sparse_csr:
+        hc = hc.coalesce().to_sparse_csr() if hc.layout == torch.sparse_coo else hc.to_sparse_csr()
+    rowptr = hc.crow_indices().to(torch.int64).contiguous()
+    cols = hc.col_indices().to(torch.int64).contiguous()
+    vals = hc.values().contiguous()
+    out = torch.empty(2, dtype=torch.float64, device=s.device)
+    nb = _i64()
+    check(lib.tcm_csr_expect_workspace_bytes(n, C.byref(nb)), "workspace")
+    ws = _workspace(nb.value, s.device)
+    check(lib.tcm_csr_expect(n, code, rowptr.data_ptr(), cols.data_ptr(), vals.data_ptr(), s.data_ptr(),
+                             out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()), "tcm_csr_expect")
+    return torch.complex(out[0], out[1])
+
+
 def sample_prob_indices(p: torch.Tensor, n: int, uniforms: torch.Tensor) -> torch.Tensor:
     """int64 [batch, shots]: inverse-CDF draws from probability vectors p
     [batch, 2^n] (normalised by their sum) for the uniforms [batch, shots]

@@ -219,3 +219,81 @@ def correlation_from_counts(index: Sequence[int], results: Any) -> torch.Tensor:
         par = par ^ ((i >> (n - 1 - j)) & 1)
     sign = 1 - 2 * par
     return torch.sum(p * sign.to(p.dtype))
+
+
+# ---------------------------------------------------------------------------
+# Pauli-string Hamiltonians as matrices (quantum.py:1163-1485)
+# ---------------------------------------------------------------------------
+
+
+def _pauli_of(ls: Any, weight: Any = None) -> Tuple[Any, int]:
+    from . import engine
+
+    codes = np.asarray(torch.as_tensor(ls).detach().cpu().numpy() if isinstance(ls, torch.Tensor) else ls)
+    codes = codes.astype(np.int8).reshape(len(codes), -1)
+    n = codes.shape[1]
+    if weight is None:
+        w = np.ones(codes.shape[0])
+    else:
+        w = np.asarray(weight.detach().cpu().numpy() if isinstance(weight, torch.Tensor) else weight)
+        w = np.real(w).astype(np.float64).reshape(-1)
+    return engine.PauliSum(n, codes, w, jit=0), n
+
+
+def PauliStringSum2COO(ls: Any, weight: Any = None, numpy: bool = False) -> Any:
+    """sum_k w_k P_k as a sparse matrix (quantum.py:1304-1356): a coalesced
+    torch sparse COO tensor on the device (entries from tcm_pauli_coo), or a
+    scipy COO matrix with ``numpy``.  Rows of ``ls`` are Pauli codes
+    0=I 1=X 2=Y 3=Z, qubit 0 most significant."""
+    from . import cons, engine
+
+    ham, _ = _pauli_of(ls, weight)
+    m = engine.pauli_coo(ham, cons.dtypestr)
+    if numpy:
+        import scipy.sparse
+
+        idx = m.indices().cpu().numpy()
+        return scipy.sparse.coo_matrix((m.values().cpu().numpy(), (idx[0], idx[1])), shape=tuple(m.shape))
+    return m
+
+
+PauliStringSum2COO_numpy = lambda ls, weight=None: PauliStringSum2COO(ls, weight, numpy=True)  # noqa: E731
+
+
+def PauliString2COO(l: Sequence[int], weight: Optional[float] = None) -> Any:
+    """One Pauli string as a sparse matrix (quantum.py:1415-1458)."""
+    return PauliStringSum2COO([list(l)], None if weight is None else [weight])
+
+
+def PauliStringSum2Dense(ls: Any, weight: Any = None, numpy: bool = False) -> Any:
+    """Dense 2^n x 2^n matrix of the sum (quantum.py:1257-1283)."""
+    m = PauliStringSum2COO(ls, weight).to_dense()
+    return m.cpu().numpy() if numpy else m
+
+
+def heisenberg_hamiltonian(g: Any, hzz: float = 1.0, hxx: float = 1.0, hyy: float = 1.0, hz: float = 0.0,
+                           hx: float = 0.0, hy: float = 0.0, sparse: bool = True, numpy: bool = False) -> Any:
+    """XXZ-type Hamiltonian on the edges of graph g with fields on its nodes
+    (quantum.py:1163-1254): for each edge ZZ, XX, YY terms (weights hzz,
+    hxx, hyy), then for each node Z, X, Y terms (hz, hx, hy); zero weights
+    are left out.  g needs ``nodes`` and ``edges`` (networkx or alike)."""
+    nodes = list(g.nodes)
+    n = len(nodes)
+    ls, w = [], []
+    for e in g.edges:
+        for code, h in ((3, hzz), (1, hxx), (2, hyy)):
+            if h != 0:
+                r = [0] * n
+                r[e[0]] = r[e[1]] = code
+                ls.append(r)
+                w.append(h)
+    for node in nodes:
+        for code, h in ((3, hz), (1, hx), (2, hy)):
+            if h != 0:
+                r = [0] * n
+                r[node] = code
+                ls.append(r)
+                w.append(h)
+    if sparse:
+        return PauliStringSum2COO(ls, w, numpy=numpy)
+    return PauliStringSum2Dense(ls, w, numpy=numpy)

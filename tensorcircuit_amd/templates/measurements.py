@@ -52,6 +52,20 @@ def parameterized_local_measurements(c: Any, structures: Any, onehot: bool = Fal
     return torch.stack(vals)
 
 
-def operator_expectation(c: Any, h: "engine.PauliSum") -> Any:
-    """<psi|H|psi> for a PauliSum (templates/measurements.py:156-174)."""
-    return engine.PauliExpFn.apply(c._state(), h)
+def operator_expectation(c: Any, h: Any) -> Any:
+    """Real <psi|H|psi> for H a PauliSum (fused Pauli kernel), a sparse
+    matrix (CSR kernel) or a dense matrix (templates/measurements.py:156-174)."""
+    if isinstance(h, engine.PauliSum):
+        return torch.real(engine.PauliExpFn.apply(c._state(), h))
+    if isinstance(h, torch.Tensor) and h.layout in (torch.sparse_coo, torch.sparse_csr):
+        return sparse_expectation(c, h)
+    w = c._state().reshape(-1, 1)
+    hd = torch.as_tensor(h).to(device=w.device, dtype=w.dtype)
+    return torch.real((torch.conj(w).T @ (hd @ w))[0, 0])
+
+
+def sparse_expectation(c: Any, hamiltonian: Any) -> Any:
+    """Real <psi|H|psi> for a sparse H (COO or CSR, e.g. from
+    quantum.PauliStringSum2COO) by the CSR kernel with fp64 accumulation
+    (templates/measurements.py:177-191)."""
+    return torch.real(engine.csr_expectation(hamiltonian, c._state(), c._nstate))
