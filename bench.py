@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--adj-tile", type=int, default=0, help="adjoint tile bits (0 = default)")
     ap.add_argument("--jit", type=int, default=1, help="1: compiled pass kernels (pass compiler), 0: generic")
     ap.add_argument("--reg-slots", type=int, default=0, help="register-resident qubits (0 = default)")
+    ap.add_argument("--mfma", type=int, default=1,
+                    help="c5: 1 = one streaming matrix-core pass per 5-qubit block, 2 = blocks grouped "
+                         "into LDS tile passes")
     ap.add_argument("--greedy", type=int, default=0,
                     help="1: first-fit pass scheduling (no search for fewer passes)")
     ap.add_argument("--no-plain", action="store_true")
@@ -287,7 +290,7 @@ def run_c5(args):
     t_stage = time.perf_counter()
     gates = workloads.brickwork(n, depth, 1005)
     blocks = fusion.fuse(gates, n)
-    p32 = engine.Program(n, blocks, 0, fuse=1, mfma=1)
+    p32 = engine.Program(n, blocks, 0, fuse=1, mfma=args.mfma, tile_bits=args.tile)
     staging = time.perf_counter() - t_stage
     st32 = torch.empty(1, 1 << n, dtype=torch.complex64, device="cuda")
     dt32, _ = _timed(lambda: p32.forward(None, 1, "complex64", out=st32), args.steps, args.warmup)
@@ -304,7 +307,8 @@ def run_c5(args):
            "ms_per_step": round(dt32 * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "c64", "data": "synthetic",
            "config": {"workload": "brickwork_haar_su2_cz", "nqubits": n, "depth": depth, "gates": len(gates),
-                      "fused_blocks": nb, "seed": 1005, "staging_s": round(staging, 3)},
+                      "fused_blocks": nb, "seed": 1005, "staging_s": round(staging, 3),
+                      "mfma_mode": args.mfma, "passes": p32.npasses},
            "fidelity_vs_c128": fid, "c128_ms": round(dt64 * 1e3, 3),
            "mfma": {"achieved": round(flops / dt32 / 1e12, 2), "peak": 157.3, "unit": "TFLOP/s",
                     "frac": round(flops / dt32 / 1e12 / 157.3, 4)},

@@ -2,6 +2,7 @@
 // table upload, workspace carving and the launch sequences for forward
 // simulation, Pauli-sum energies and the adjoint reverse sweep.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -210,7 +211,8 @@ int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const 
       if (S.ckpt_after[c] < 0) launch_copy<T>(ckpt_ptr(c), state, ck_stride, st);
   const int np = (int)S.passes.size();
   for (int pi = 0; pi < np; ++pi) {
-    const bool tiled = S.k > 0 && S.dpass[pi].plain_op < 0;
+    const bool mfma = S.k > 0 && S.dpass[pi].mfma;
+    const bool tiled = S.k > 0 && S.dpass[pi].plain_op < 0 && !mfma;
     void* ckout = nullptr;
     std::vector<size_t> extra;
     if (ckpts)
@@ -219,7 +221,13 @@ int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const 
           if (!ckout && tiled) ckout = ckpt_ptr(c);
           else extra.push_back(c);
         }
-    if (tiled) {
+    if (mfma) {
+      if constexpr (std::is_same<T, float>::value)
+        launch_mfma_tile(S.dpass[pi], (const DPass*)D.dpass, pi, (const DOp*)D.dop, (const int32_t*)D.ukbits,
+                         (const uint32_t*)D.runoff, D.ctab, state, n, batch, st);
+      else
+        return set_error(TCM_E_UNSUPPORTED, "matrix-core passes are complex64 only");
+    } else if (tiled) {
       if (!S.jit.empty() && S.jit[pi].kernel >= 0) {
         if (int rc = jit_launch_fwd(S, D, pi, sizeof(T) == 8, mtab, D.ctab, state, batch, n, ckout, st)) return rc;
       } else if (S.R == 4) {
@@ -671,6 +679,7 @@ int forward_multi_impl(const tcm_multi_desc* descs, int ndesc, int dtype, int in
         continue;
       }
       if (S.R != 4) return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
+      if (S.dpass[r].mfma) return set_error(TCM_E_UNSUPPORTED, "matrix-core passes are not run in multi launches");
       const DPass& dp = S.dpass[r];
       MultiJob& q = J.job[J.njobs++];
       q.dpass = D[i]->dpass;

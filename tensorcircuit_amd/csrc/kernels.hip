@@ -1109,6 +1109,95 @@ __global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restri
   }
 }
 
+// One tile pass of dense 5-qubit blocks on FP32 matrix cores.  The tile
+// (2^k amplitudes whose index bits are the pass's qubits) is staged in LDS;
+// each block multiplies the tile, viewed as 32 rows (the block's 5 tile
+// bits, first bit most significant) x 2^(k-5) columns (the other tile bits),
+// by its 32x32 complex matrix: every wave takes 32-column chunks and forms
+// the complex product as four real v_mfma_f32_32x32x2f32 streams over the 16
+// K-steps, reading its B operands from LDS and writing its result columns
+// back (waves own disjoint columns, so only the block boundary needs a
+// barrier).  512 threads, dynamic LDS of 8 << k bytes; grid = batch << (n - k).
+__global__ void __launch_bounds__(512) k_mfma_tile(const DPass* __restrict__ passes, int pi,
+                                                   const DOp* __restrict__ dop, const int32_t* __restrict__ ukbits,
+                                                   const uint32_t* __restrict__ runoff, const float* __restrict__ ct,
+                                                   float2* __restrict__ st, int64_t sstride) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tile = reinterpret_cast<float2*>(smem);
+  const DPass* P = uniform_ptr(passes + pi);
+  const int k = P->k, L = P->L, nbl = P->nblk_log;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
+  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
+  uint64_t base = 0;
+  for (int i = 0; i < nbl; ++i)
+    if ((blk >> i) & 1) base |= 1ull << P->blk_bits[i];
+  float2* s = st + b * sstride + base;
+  const uint32_t* ro = runoff + P->runoff0;
+  const int per = (1 << k) / nthr;
+  const uint32_t dlo = tile_dep(ro, L, tid);
+  const uint32_t slo = swz(tid);
+  for (int it = 0; it < per; ++it) tile[swz(it * nthr) ^ slo] = s[tile_dep(ro, L, it * nthr) + dlo];
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
+  const int ai = lane & 31, ak = lane >> 5;
+  const int chunks = 1 << (k - 10);
+  for (int o = 0; o < P->nmop; ++o) {
+    const DOp op = dop[P->mop0 + o];
+    const float* U = ct + (-op.moff - 1);
+    // 5 tile-local gate bits, then the swizzled tile offsets of the 32 rows
+    const int32_t* ub = ukbits + op.mem0;
+    uint32_t bm = 0;
+    for (int i = 0; i < 5; ++i) bm |= 1u << ub[i];
+    const int32_t* rows = ub + 5;
+    float ur[16], ui[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int kk = 2 * q + ak;
+      ur[q] = U[2 * (ai * 32 + kk)];
+      ui[q] = U[2 * (ai * 32 + kk) + 1];
+    }
+    for (int chunk = wave; chunk < chunks; chunk += nwaves) {
+      // column index -> the tile bits outside the block, ascending; the
+      // swizzle is linear over XOR, so swz(cb | row) = swz(cb) ^ swz(row)
+      const uint32_t col = (uint32_t)chunk * 32 + ai;
+      uint32_t cb = 0;
+      for (int t = 0, j = 0; t < k; ++t)
+        if (!((bm >> t) & 1)) {
+          if ((col >> j) & 1) cb |= 1u << t;
+          ++j;
+        }
+      const uint32_t xc = swz(cb);
+      float2 x[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] = tile[xc ^ (uint32_t)(ak ? rows[2 * q + 1] : rows[2 * q])];
+      tcm_v16f dr = {0}, di = {0};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        dr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].x, dr, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[q], x[q].y, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].y, di, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[q], x[q].x, di, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + (r & 3);
+        tile[xc ^ (uint32_t)(ak ? rows[row + 4] : rows[row])] = make_float2(dr[r], di[r]);
+      }
+    }
+    __syncthreads();
+  }
+  for (int it = 0; it < per; ++it) s[tile_dep(ro, L, it * nthr) + dlo] = tile[swz(it * nthr) ^ slo];
+}
+
+void launch_mfma_tile(const DPass& P, const DPass* passes_dev, int pi, const DOp* dop_dev, const int32_t* ukbits_dev,
+                      const uint32_t* runoff_dev, const void* ctab, void* state, int n, int64_t batch,
+                      hipStream_t st) {
+  const int64_t blocks = batch << P.nblk_log;
+  hipLaunchKernelGGL(k_mfma_tile, dim3((unsigned)blocks), dim3(512), (size_t)8 << P.k, st, passes_dev, pi, dop_dev,
+                     ukbits_dev, runoff_dev, (const float*)ctab, (float2*)state, (int64_t)1 << n);
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------

@@ -424,7 +424,7 @@ int adj_partials(int dt, const DPass& dp) {
 }
 
 bool pass_supported(const Sched& S, const DPass& dp) {
-  if (dp.plain_op >= 0 || dp.k <= 0) return false;
+  if (dp.plain_op >= 0 || dp.k <= 0 || dp.mfma) return false;
   if ((1 << dp.tthreads_log) < 64 || dp.tthreads_log > 10) return false;
   for (int g = 0; g < dp.ngroups; ++g) {
     const DGroup& dg = S.dgroup[dp.group0 + g];

@@ -749,7 +749,32 @@ static void tile_run(const Program& P, Sched& S, const std::vector<int>& items, 
   }
 }
 
-static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant) {
+// Consecutive dense 5-qubit gates (matrix-core blocks) grouped into tile
+// passes: each pass stages a tile of kM bits (the low Lwant index bits plus
+// the blocks' qubits) and applies its blocks in order from LDS.
+static void mfma_tile_run(const Program& P, Sched& S, const std::vector<int>& items, uint64_t lmask, int kM) {
+  const int n = P.n;
+  const int cap = kM - popc64(lmask);
+  std::vector<uint64_t> used;
+  auto clusters = cluster(
+      items, [&](int gi) { return gate_mask(P.gates[gi]); }, lmask, cap, [](int) { return false; }, &used,
+      !P.opts.greedy);
+  for (size_t ci = 0; ci < clusters.size(); ++ci) {
+    HPass hp;
+    hp.mfma = true;
+    hp.gates = clusters[ci];
+    uint64_t tmask = lmask | used[ci];
+    for (int b = 0; b < n && popc64(tmask) < kM; ++b) tmask |= (1ull << b);
+    for (int b = 0; b < n; ++b)
+      if (tmask >> b & 1) hp.tile.push_back(b);
+    int L = 0;
+    while (L < (int)hp.tile.size() && hp.tile[L] == L) ++L;
+    hp.L = L;
+    S.passes.push_back(hp);
+  }
+}
+
+static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant, bool mfma_tiles) {
   const int n = P.n;
   k = std::min(k, n);
   S.k = k;
@@ -768,29 +793,88 @@ static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant) {
     int g0 = bounds[sgi], g1 = bounds[sgi + 1];
     if (sgi > 0) S.ckpt_after.push_back((int)S.passes.size() - 1);
     if (g1 <= g0) continue;
-    // runs of <=2-qubit gates are tiled; a >=3-qubit gate is a plain pass
-    std::vector<int> run;
+    // runs of <=2-qubit gates are tiled; a >=3-qubit gate is a plain pass,
+    // except runs of dense 5-qubit gates on matrix cores (complex64), which
+    // share tile passes of 2^13 amplitudes
+    std::vector<int> run, mrun;
+    const int kM = std::min(P.opts.tile_bits > 0 ? std::max(10, std::min(P.opts.tile_bits, 13)) : 13, n);
     auto flush = [&]() {
-      if (run.empty()) return;
-      tile_run(P, S, run, lmask, cap);
+      if (!run.empty()) tile_run(P, S, run, lmask, cap);
       run.clear();
     };
+    auto mflush = [&]() {
+      if (!mrun.empty()) mfma_tile_run(P, S, mrun, lmask, kM);
+      mrun.clear();
+    };
     for (int gi = g0; gi < g1; ++gi) {
-      if (P.gates[gi].nq >= 3) {
+      const LGate& g = P.gates[gi];
+      if (mfma_tiles && g.nq == 5 && g.okind == OK_UK && g.cslot >= 0 && n >= kM && kM >= 10) {
         flush();
+        mrun.push_back(gi);
+      } else if (g.nq >= 3) {
+        flush();
+        mflush();
         HPass hp;
         hp.gates.push_back(gi);
         S.passes.push_back(hp);
       } else {
+        mflush();
         run.push_back(gi);
       }
     }
     flush();
+    mflush();
   }
   // flatten to device descriptors
   for (HPass& hp : S.passes) {
     DPass dp{};
     dp.plain_op = -1;
+    if (hp.mfma) {
+      dp.k = (int)hp.tile.size();
+      dp.L = hp.L;
+      dp.nblk_log = n - dp.k;
+      dp.tthreads_log = 9;
+      dp.mfma = 1;
+      dp.gop_list0 = (int)S.gop_list.size();
+      dp.runoff0 = (int)S.runoff.size();
+      int nb = 0;
+      for (int b = 0; b < n; ++b)
+        if (std::find(hp.tile.begin(), hp.tile.end(), b) == hp.tile.end()) dp.blk_bits[nb++] = b;
+      for (int r = 0; r < (1 << (dp.k - dp.L)); ++r) {
+        uint32_t off = 0;
+        for (int i = 0; i < dp.k - dp.L; ++i)
+          if (r >> i & 1) off |= 1u << hp.tile[dp.L + i];
+        S.runoff.push_back(off);
+      }
+      std::vector<int> local_of_bit(n, -1);
+      for (int i = 0; i < dp.k; ++i) local_of_bit[hp.tile[i]] = i;
+      dp.mop0 = (int)S.dop.size();
+      for (int gi : hp.gates) {
+        const LGate& g = P.gates[gi];
+        DOp op{};
+        op.kind = OK_UK;
+        op.s0 = op.s1 = -1;
+        op.moff = -g.cslot - 1;
+        op.gen = GN_NONE;
+        op.gop = -1;
+        op.gloc = -1;
+        op.nmem = 5;
+        op.mem0 = (int)S.ukbits.size();
+        op.flags = OP_F_MFMA;
+        for (int i = 0; i < 5; ++i) S.ukbits.push_back(local_of_bit[g.bits[i]]);
+        // then the swizzled tile offsets of the 32 gate rows
+        for (int r = 0; r < 32; ++r) {
+          uint32_t x = 0;
+          for (int i = 0; i < 5; ++i)
+            if ((r >> (4 - i)) & 1) x |= 1u << local_of_bit[g.bits[i]];
+          S.ukbits.push_back((int32_t)swz_host(x));
+        }
+        S.dop.push_back(op);
+      }
+      dp.nmop = (int)hp.gates.size();
+      S.dpass.push_back(dp);
+      continue;
+    }
     if (hp.tile.empty()) {
       // a >=3-qubit gate applied by the plain kernel (no gradient)
       const LGate& g = P.gates[hp.gates[0]];
@@ -936,7 +1020,7 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
       Sched& S = P->sched[dt][mode];
       int k = mode == 0 ? kf : ka;
       if (fused && std::min(k, n) >= R + 6 && k >= std::min(n, Lw + 5)) {
-        sched_fused(*P, S, k, R, Lw);
+        sched_fused(*P, S, k, R, Lw, dt == 0 && mode == 0 && o.mfma == 2);
       } else {
         sched_plain(*P, S);
       }
@@ -956,7 +1040,7 @@ static void json_sched(std::ostringstream& os, const Program& P, const Sched& S)
     const HPass& hp = S.passes[pi];
     os << (pi ? "," : "") << "{\"tile\":[";
     for (size_t i = 0; i < hp.tile.size(); ++i) os << (i ? "," : "") << hp.tile[i];
-    os << "],\"L\":" << hp.L << ",\"gates\":[";
+    os << "],\"L\":" << hp.L << ",\"mfma\":" << (hp.mfma ? 1 : 0) << ",\"gates\":[";
     for (size_t i = 0; i < hp.gates.size(); ++i) os << (i ? "," : "") << hp.gates[i];
     os << "],\"groups\":[";
     for (size_t q = 0; q < hp.groups.size(); ++q) {

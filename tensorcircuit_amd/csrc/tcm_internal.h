@@ -99,6 +99,9 @@ struct DPass {
   int32_t plain_op;   // >= 0: not a tile pass but one plain gate (Sched::dop index)
   int32_t full;       // 1: the pass holds dense 4x4 ops / generators (full kernel)
   int32_t blk_bits[32];  // global bit positions of the block bits (ascending)
+  int32_t mfma;       // 1: a tile pass of dense 5-qubit blocks on FP32 matrix cores
+  int32_t mop0, nmop; // its blocks: Sched::dop entries (OK_UK, tile-local bits)
+  int32_t pad;
 };
 
 // Matrix recipes evaluated on the device per batch item (prep kernel).
@@ -155,6 +158,7 @@ struct HPass {
   int32_t L = 0;
   std::vector<HGroup> groups;
   std::vector<int32_t> gates;   // all gates of the pass, program order
+  bool mfma = false;            // dense 5-qubit blocks on matrix cores (no groups)
 };
 
 struct DeviceTables;  // per-device uploaded tables (kernels.hip)

@@ -568,7 +568,7 @@ def test_mfma_dense5_vs_oracle(gpu, n):
     s0 = rng.standard_normal(2**n) + 1j * rng.standard_normal(2**n)
     s0 /= np.linalg.norm(s0)
     ref = sv.run_ops(n, ops, inputs=s0)
-    for mfma in (1, 0):
+    for mfma in (2, 1, 0):
         prog = engine.Program(n, specs, 0, fuse=1, mfma=mfma)
         st, _ = prog.forward(None, 2, "complex64", inputs=torch.as_tensor(np.stack([s0, s0])).to(torch.complex64))
         np.testing.assert_allclose(np_of(st)[1], ref, atol=2e-6)
@@ -591,6 +591,11 @@ def test_brickwork_fused_mfma_vs_c128(gpu):
         s64, _ = p64.forward(None, 1, "complex128")
         fid = abs(complex(engine.inner(s64, s32.to(torch.complex128), n)[0])) ** 2
         assert abs(fid - 1.0) < 1e-5, fid
+        # the same blocks grouped into LDS tile passes
+        pt = engine.Program(n, blocks, 0, fuse=1, mfma=2)
+        assert pt.npasses < p32.npasses
+        st_, _ = pt.forward(None, 1, "complex64")
+        np.testing.assert_allclose(np_of(st_), np_of(s32), atol=2e-6)
         if n == 14:
             ref = sv.zero_state(n)
             for g in gates:
