@@ -297,13 +297,91 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
   // the item list is long)
   int beam = 8, ncand = 8;
   if (!climb || items.size() > 3000) beam = ncand = 1;
+  std::vector<std::vector<int>> shorter_out;
   std::vector<Node> front(1);
   front[0].rest = items;
   prepare(front[0]);
   std::vector<int> taken, rest;
+  // exact search for a schedule one cluster shorter than the beam's, over
+  // the climbed set, its best neighbours and every window of `cap`
+  // consecutive free bits (a chain-structured circuit's staircase windows
+  // are such windows); bounded by a node budget
+  bool any_solo = false;
+  for (int it : items) any_solo |= gsolo[it] != 0;
+  auto shorter = [&](const Node& done) -> bool {
+    const int target = (int)done.clusters.size() - 1;
+    if (!climb || any_solo || target < 1 || target > 5 || items.size() > 4000) return false;
+    uint64_t allbits = 0;
+    for (int it : items) allbits |= gmask[it];
+    allbits &= ~fixed;
+    std::vector<int> fb;
+    for (int b = 0; b < 64; ++b)
+      if (allbits >> b & 1) fb.push_back(b);
+    std::vector<uint64_t> windows;
+    for (size_t i = 0; i + cap <= fb.size(); ++i) {
+      uint64_t w = 0;
+      for (int j = 0; j < cap; ++j) w |= 1ull << fb[i + j];
+      windows.push_back(w);
+    }
+    int budget = 4000;
+    std::vector<std::vector<int>> cl;
+    std::vector<uint64_t> sets;
+    std::function<bool(const std::vector<int>&, int)> dfs = [&](const std::vector<int>& rem, int left) -> bool {
+      if (rem.empty()) return true;
+      if (left == 0 || --budget < 0) return false;
+      ClusterScorer loc;
+      loc.cap = cap;
+      loc.fixed = fixed;
+      loc.set_items(rem, gmask, gsolo);
+      uint64_t u = 0;
+      for (int it : rem) u |= gmask[it];
+      u &= ~fixed;
+      std::vector<uint64_t> cands;
+      if (popc64(u) <= cap) {
+        cands.push_back(u);  // one cluster takes everything left
+      } else {
+        if (left == 1) return false;
+        int best = 0;
+        std::vector<std::pair<int, uint64_t>> near;
+        const uint64_t c0 = loc.climb(loc.first_fit(), &best, &near);
+        std::vector<std::pair<int, uint64_t>> all(near);
+        all.push_back({best, c0});
+        std::vector<int> wc(windows.size(), 0);
+        for (size_t o = 0; o < windows.size(); o += 64)
+          loc.score(windows.data() + o, (int)std::min<size_t>(64, windows.size() - o), wc.data() + o);
+        for (size_t i = 0; i < windows.size(); ++i) all.push_back({wc[i], windows[i]});
+        std::sort(all.begin(), all.end(), [](const std::pair<int, uint64_t>& a, const std::pair<int, uint64_t>& b) {
+          return a.first > b.first || (a.first == b.first && a.second < b.second);
+        });
+        for (const auto& pr : all) {
+          if (pr.first <= 0) break;
+          if (std::find(cands.begin(), cands.end(), pr.second) == cands.end()) cands.push_back(pr.second);
+          if (cands.size() >= 24) break;
+        }
+      }
+      std::vector<int> tk, rs;
+      for (uint64_t c : cands) {
+        loc.split(c, tk, rs);
+        if (tk.empty()) continue;
+        uint64_t usedb = 0;
+        for (int it : tk) usedb |= gmask[it];
+        cl.push_back(tk);
+        sets.push_back(c & usedb);
+        if (dfs(rs, left - 1)) return true;
+        cl.pop_back();
+        sets.pop_back();
+      }
+      return false;
+    };
+    if (!dfs(items, target)) return false;
+    if (used_out) *used_out = sets;
+    shorter_out = cl;
+    return true;
+  };
   while (true) {
     for (const Node& n : front)
       if (n.rest.empty()) {
+        if (shorter(n)) return shorter_out;
         if (used_out) *used_out = n.sets;
         return n.clusters;
       }
