@@ -30,11 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # scheduling).  Commands and per-counter figures:
 # profiles/tcm_adj_passes_hea24_b32.md (compiled passes, searched schedule)
 # and profiles/k_fused_adj_hea24_b32.md (generic kernels, first-fit schedule).
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 17564078387}
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 17315946826}
 # VALU wave-instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, mean over
 # the adjoint / forward pass launches of one sweep), same workload and key;
 # profiles/tcm_adj_passes_hea24_b32.md.
-MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (3599708979, 1334339174)}
+MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (4609048576, 1704755200)}
 # MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
 # every 4 cycles at 2.4 GHz
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
