@@ -11,7 +11,7 @@ from . import _lib  # noqa: F401  (fails loudly without libtcm.so)
 from . import cons, experimental, gates, interfaces, quantum, templates, torchnn, translation
 from .backend import backend
 from .circuit import Circuit, expectation
-from .cons import get_backend, runtime_dtype, set_backend, set_dtype, set_jit
+from .cons import get_backend, get_contractor, runtime_dtype, set_backend, set_contractor, set_dtype, set_jit
 from .engine import PauliSum, Program
 from .gates import array_to_tensor, num_to_tensor
 from .interfaces import torch_interface

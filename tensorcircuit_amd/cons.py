@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import contextlib
 import os
-from typing import Iterator, Optional
+from typing import Any, Iterator, Optional
 
 dtypestr = "complex64"
 rdtypestr = "float32"
@@ -67,3 +67,24 @@ def set_backend(backend: Optional[str] = None):
 
 
 get_backend = set_backend
+
+_CONTRACTOR = {"method": "greedy", "optimizer": None}
+
+
+def set_contractor(method: Optional[str] = None, optimizer: Any = None, memory_limit: Any = None,
+                   opt_conf: Any = None, set_global: bool = True, contraction_info: bool = False,
+                   debug_level: int = 0, preprocessing: bool = False) -> Any:
+    """Accepted for compatibility with the reference's contractor hook
+    (cons.py:732-830); the statevector engine applies gates in program
+    order through fused passes and needs no contraction path, so the choice
+    is recorded and has no effect.  Returns the recorded setting."""
+    if set_global:
+        _CONTRACTOR.update({"method": method or "greedy", "optimizer": optimizer})
+    return dict(_CONTRACTOR)
+
+
+def get_contractor(method: Optional[str] = None, optimizer: Any = None, memory_limit: Any = None,
+                   opt_conf: Any = None, contraction_info: bool = False, debug_level: int = 0,
+                   preprocessing: bool = False) -> Any:
+    return set_contractor(method, optimizer, memory_limit, opt_conf, False, contraction_info, debug_level,
+                          preprocessing)

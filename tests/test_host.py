@@ -317,3 +317,58 @@ def test_json_round_trip_and_qsim_parse(tmp_path):
     names = [x["name"] for x in c3.to_qir()]
     assert names == ["h", "rx", "cz", "rz", "iswap", "cphase", "wroot", "phase"]
     np.testing.assert_allclose(tr._matrix(c3.to_qir()[4]), tc.gates.iswap_matrix(-0.5), atol=1e-12)
+
+
+def test_pass_schedule_search():
+    """The pass scheduler on the host: first fit needs 44 tile passes for the
+    24-qubit 20-layer HEA with checkpoints every 5 layers; the search (hill
+    climbing, beam, bounded exact search) finds the 4-pass staircase per
+    segment, 16 in all, and every gate lands in exactly one pass in an order
+    that respects each qubit's gate sequence."""
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L = 24, 20
+    c = Circuit(n)
+    hea_layers(c, np.zeros((2 * L, n), np.float32), L)
+    specs, thetas = c._lower()
+    greedy = engine.Program(n, specs, len(thetas), fuse=1, ckpt_every=5, jit=1, greedy=1)
+    search = engine.Program(n, specs, len(thetas), fuse=1, ckpt_every=5, jit=1)
+    assert greedy.npasses == 44
+    assert search.npasses == 16
+    for prog in (greedy, search):
+        d = prog.describe("complex64", 0)
+        order = [g for p in d["sched"]["passes"] for g in p["gates"]]
+        assert d["ngates"] == 1420  # layer barriers are not gates
+        assert sorted(order) == list(range(d["ngates"]))
+        # per qubit, the gates appear in program order
+        last = {}
+        for gi in order:
+            for q in specs_qubits(specs, gi):
+                assert last.get(q, -1) < gi
+                last[q] = gi
+
+
+def specs_qubits(specs, gi):
+    """Qubits of lowered gate gi (barriers removed)."""
+    from tensorcircuit_amd import _lib
+
+    real = [s for s in specs if s.kind != _lib.G_BARRIER]
+    return real[gi].qubits
+
+
+def test_mfma_tile_pass_schedule():
+    """Matrix-core tile passes (mfma=2) group the 76 fused 5-qubit blocks of
+    the 30-qubit depth-20 brickwork into 34 passes of 2^13 amplitudes."""
+    from tensorcircuit_amd import engine, fusion, workloads
+
+    n = 30
+    blocks = fusion.fuse(workloads.brickwork(n, 20, 1005), n)
+    assert len(blocks) == 76
+    p1 = engine.Program(n, blocks, 0, fuse=1, mfma=1)
+    p2 = engine.Program(n, blocks, 0, fuse=1, mfma=2)
+    assert p1.npasses == 76
+    assert p2.npasses == 34
+    sched = p2.describe("complex64", 0)["sched"]
+    assert all(p["mfma"] == 1 and len(p["tile"]) == 13 for p in sched["passes"])
+    assert sorted(g for p in sched["passes"] for g in p["gates"]) == list(range(76))
