@@ -30,11 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # scheduling).  Commands and per-counter figures:
 # profiles/tcm_adj_passes_hea24_b32.md (compiled passes, searched schedule)
 # and profiles/k_fused_adj_hea24_b32.md (generic kernels, first-fit schedule).
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 17315946826}
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 16484712844}
 # VALU wave-instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, mean over
 # the adjoint / forward pass launches of one sweep), same workload and key;
 # profiles/tcm_adj_passes_hea24_b32.md.
-MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (4609048576, 1704755200)}
+MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (4252221440, 1579745280)}
 # MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
 # every 4 cycles at 2.4 GHz
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
@@ -159,7 +159,7 @@ def main():
         "config": {"workload": "hea_tfim_energy_grad", "nqubits": n, "layers": L, "params_per_set": P,
                    "batch": B, "ckpt_every_layers": args.ckpt, "h": 1.0, "seed": 1004,
                    "fwd_passes": prog.npasses, "staging_s": round(staging_s, 3), "jit": args.jit,
-                   "jit_compile_s": round(jit_s, 2), "reg_slots": args.reg_slots or 4,
+                   "jit_compile_s": round(jit_s, 2), "reg_slots": prog.describe("complex64", 0)["sched"]["R"],
                    "sched": "first-fit" if args.greedy else "search"},
     }
 
@@ -215,7 +215,7 @@ def main():
         fwd_gbs = fwd_bytes / (fwd_launch_ms * 1e-3) / 1e9
         traffic = None
         valu = None
-        if args.adj_tile == 0 and args.tile == 0 and args.reg_slots in (0, 4):
+        if args.adj_tile == 0 and args.tile == 0 and args.reg_slots == 0:
             key = (n, L, B, args.ckpt, args.jit, args.greedy)
             traffic = MEASURED_ADJ_TRAFFIC.get(key)
             valu = MEASURED_VALU_INSTS.get(key)

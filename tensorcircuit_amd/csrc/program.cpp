@@ -1082,7 +1082,10 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
     return set_error(TCM_E_UNSUPPORTED, "reg_slots=%d: 4 register-resident qubits (3 to 5 with jit)", o.reg_slots);
   }
   for (int dt = 0; dt < 2; ++dt) {
-    const int R = o.reg_slots ? o.reg_slots : 4;
+    // register-resident qubits: 5 for compiled complex64 passes (32
+    // amplitudes per thread: fewer LDS exchanges per gate; measured faster
+    // on the 24-qubit HEA despite 2 waves per SIMD in the adjoint), else 4
+    const int R = o.reg_slots ? o.reg_slots : ((o.jit && dt == 0) ? 5 : 4);
     const int Lw = dt == 0 ? 4 : 3;
     // defaults measured on MI355X: 32 KiB forward tiles (several blocks per
     // CU overlap HBM and compute); the generic adjoint holds two tiles.

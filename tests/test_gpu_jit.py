@@ -80,7 +80,7 @@ def _terms(rng, n, nterms):
 
 @pytest.mark.parametrize("n", [10, 13])
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
-@pytest.mark.parametrize("R", [4, 3])
+@pytest.mark.parametrize("R", [5, 4, 3])
 def test_jit_energy_grad_random_vs_oracle(gpu, n, dtype, R):
     """All gate kinds and generator kinds of the fused passes, compiled."""
     import tensorcircuit_amd as tc
@@ -106,7 +106,7 @@ def test_jit_energy_grad_random_vs_oracle(gpu, n, dtype, R):
     np.testing.assert_allclose(np_of(st).reshape(-1), ref, atol=4e-5 if dtype == "complex64" else 1e-11)
 
 
-@pytest.mark.parametrize("R", [4, 3])
+@pytest.mark.parametrize("R", [5, 4, 3])
 def test_jit_matches_generic_hea_batch(gpu, R):
     """20-qubit HEA, batch 4, checkpoints every 2 layers: compiled passes
     against the generic fused kernels and the oracle."""
