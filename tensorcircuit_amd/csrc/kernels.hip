@@ -1083,16 +1083,29 @@ __global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restri
   const int64_t ncol = 1ll << (n - 5);
   const int64_t chunks_per_state = ncol >> 5;
   const int64_t nchunks = chunks_per_state * batch;
-  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+  // chunk ch: 32 consecutive columns of one state; the next chunk's loads
+  // are issued before this chunk's matrix-core work (software pipelining)
+  auto chunk_ptr = [&](int64_t ch) {
     const int64_t b = ch / chunks_per_state;
     const uint64_t c = ((uint64_t)(ch % chunks_per_state) << 5) + (lane & 31);
     uint64_t base = c;
 #pragma unroll
     for (int i = 0; i < 5; ++i) base = ins0(base, sorted[i]);
-    float2* s = st + b * sstride + base;
-    float2 x[16];
+    return st + b * sstride + base;
+  };
+  if (wave >= nchunks) return;
+  float2* s = chunk_ptr(wave);
+  float2 x[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) x[q] = s[rowoff(2 * q + ak)];
+  for (int q = 0; q < 16; ++q) x[q] = s[rowoff(2 * q + ak)];
+  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+    const bool more = ch + nwaves < nchunks;
+    float2* sn = more ? chunk_ptr(ch + nwaves) : s;
+    float2 xn[16];
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) xn[q] = sn[rowoff(2 * q + ak)];
+    }
     tcm_v16f dr = {0}, di = {0};
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -1105,6 +1118,11 @@ __global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restri
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r >> 2) + 4 * ak + (r & 3);
       s[rowoff(row)] = make_float2(dr[r], di[r]);
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] = xn[q];
+      s = sn;
     }
   }
 }
