@@ -222,10 +222,12 @@ int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const 
           else extra.push_back(c);
         }
     if (mfma) {
-      if constexpr (std::is_same<T, float>::value)
-        launch_mfma_tile(S.dpass[pi], (const DPass*)D.dpass, pi, (const DOp*)D.dop, (const int32_t*)D.ukbits,
-                         (const uint32_t*)D.runoff, D.ctab, state, n, batch, st);
-      else
+      if constexpr (std::is_same<T, float>::value) {
+        if (!launch_mfma_tile(S.dpass[pi], (const DPass*)D.dpass, pi, (const DOp*)D.dop, (const int32_t*)D.ukbits,
+                              (const uint32_t*)D.runoff, D.ctab, state, n, batch, st))
+          return set_error(TCM_E_UNSUPPORTED, "matrix-core tile pass %d: %d blocks on %d tile bits not supported", pi,
+                           S.dpass[pi].nmop, S.dpass[pi].k);
+      } else
         return set_error(TCM_E_UNSUPPORTED, "matrix-core passes are complex64 only");
     } else if (tiled) {
       if (!S.jit.empty() && S.jit[pi].kernel >= 0) {

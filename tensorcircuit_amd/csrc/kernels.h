@@ -148,8 +148,10 @@ struct MeasureStep {
   int32_t first;     // 1 on the first step: no bits fixed yet
   int32_t run[34];   // counter bits src .. src+len-1 -> index bits dst ..
 };
-// a tile pass of dense 5-qubit blocks on FP32 matrix cores (complex64)
-void launch_mfma_tile(const DPass& P, const DPass* passes_dev, int pi, const DOp* dop_dev, const int32_t* ukbits_dev,
+// a tile pass of dense 5-qubit blocks on FP32 matrix cores (complex64);
+// false if the pass does not fit the kernel (tile bits 10..13, at most
+// kMfmaTileMaxOps blocks)
+bool launch_mfma_tile(const DPass& P, const DPass* passes_dev, int pi, const DOp* dop_dev, const int32_t* ukbits_dev,
                       const uint32_t* runoff_dev, const void* ctab, void* state, int n, int64_t batch,
                       hipStream_t st);
 template <typename T>

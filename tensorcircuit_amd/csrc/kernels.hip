@@ -9,6 +9,7 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "kernels.h"
@@ -1044,6 +1045,9 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 // ---------------------------------------------------------------------------
 typedef float tcm_v16f __attribute__((ext_vector_type(16)));
+// two complex64 amplitudes (one 16-byte access; a native vector, so copies
+// stay in registers)
+typedef float tcm_v4f __attribute__((ext_vector_type(4)));
 
 template <bool DAG>
 __global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restrict__ ukbits,
@@ -1127,93 +1131,226 @@ __global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restri
   }
 }
 
-// One tile pass of dense 5-qubit blocks on FP32 matrix cores.  The tile
-// (2^k amplitudes whose index bits are the pass's qubits) is staged in LDS;
-// each block multiplies the tile, viewed as 32 rows (the block's 5 tile
-// bits, first bit most significant) x 2^(k-5) columns (the other tile bits),
-// by its 32x32 complex matrix: every wave takes 32-column chunks and forms
-// the complex product as four real v_mfma_f32_32x32x2f32 streams over the 16
-// K-steps, reading its B operands from LDS and writing its result columns
-// back (waves own disjoint columns, so only the block boundary needs a
-// barrier).  512 threads, dynamic LDS of 8 << k bytes; grid = batch << (n - k).
+// f(0), f(1), ..., f(N - 1) with compile-time indices (register arrays
+// indexed by them stay in registers)
+template <int N, typename F>
+__device__ __forceinline__ void unrolled(F&& f) {
+  if constexpr (N > 0) {
+    unrolled<N - 1>(f);
+    f(std::integral_constant<int, N - 1>{});
+  }
+}
+
+// One tile pass of dense 5-qubit blocks on FP32 matrix cores.  A tile is the
+// 2^K amplitudes whose index bits are the pass's tile bits; each block
+// multiplies the tile, viewed as 32 rows (the block's 5 tile bits, first bit
+// most significant) x 2^(K-5) columns (the other tile bits, ascending), by
+// its 32x32 complex matrix U.  Wave w takes the 32-column chunk w and forms
+// Y = U X from three real products (Gauss's complex multiplication):
+//   P1 = Ur Xr,  P2 = Ui Xi,  P3 = (Ur + Ui)(Xr + Xi),
+//   Yr = P1 - P2,  Yi = P3 - P1 - P2,
+// 48 v_mfma_f32_32x32x2f32 per chunk instead of the 64 of four products.
+// Waves own disjoint columns, so only the block boundary needs a barrier.
+// The kernel is persistent: a block stages the A operands of all the pass's
+// blocks in LDS once, then runs tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+// and loads the next tile's amplitudes into registers while the matrix
+// cores work on the current one.  Row offsets are XORs of the 5 gate bits'
+// swizzled offsets (the swizzle is linear over XOR).  512 threads; LDS =
+// 8 << K bytes (tile) + nmop * (8 KiB A operands, [block][K-step][lane], +
+// the gate-bit and column offsets).
+template <int K>
 __global__ void __launch_bounds__(512) k_mfma_tile(const DPass* __restrict__ passes, int pi,
                                                    const DOp* __restrict__ dop, const int32_t* __restrict__ ukbits,
                                                    const uint32_t* __restrict__ runoff, const float* __restrict__ ct,
-                                                   float2* __restrict__ st, int64_t sstride) {
+                                                   float2* __restrict__ st, int64_t sstride, int64_t ntiles) {
+  // each thread moves PER2 pairs of adjacent amplitudes (16-byte accesses)
+  constexpr int NTHR = 512, PER2 = (1 << K) / (2 * NTHR), CHUNKS = 1 << (K - 10), NCOL = 1 << (K - 5);
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tile = reinterpret_cast<float2*>(smem);
   const DPass* P = uniform_ptr(passes + pi);
-  const int k = P->k, L = P->L, nbl = P->nblk_log;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const uint64_t blk = blockIdx.x & ((1ull << nbl) - 1);
-  const uint64_t b = (uint64_t)blockIdx.x >> nbl;
-  uint64_t base = 0;
-  for (int i = 0; i < nbl; ++i)
-    if ((blk >> i) & 1) base |= 1ull << P->blk_bits[i];
-  float2* s = st + b * sstride + base;
+  const int L = P->L, nbl = P->nblk_log, nmop = P->nmop;
+  float2* amat = tile + (1 << K);
+  // per block: the swizzled offsets of its 5 gate bits, then the swizzled
+  // offsets of its NCOL columns
+  uint32_t* ebits = reinterpret_cast<uint32_t*>(amat + nmop * 1024);
+  uint32_t* xcol = ebits + nmop * 8;
+  const int tid = threadIdx.x;
+  int64_t t = blockIdx.x;
+  if (t >= ntiles) return;
+  const DOp* ops = dop + P->mop0;
   const uint32_t* ro = runoff + P->runoff0;
-  const int per = (1 << k) / nthr;
-  const uint32_t dlo = tile_dep(ro, L, tid);
-  const uint32_t slo = swz(tid);
-  for (int it = 0; it < per; ++it) tile[swz(it * nthr) ^ slo] = s[tile_dep(ro, L, it * nthr) + dlo];
-  __syncthreads();
-  const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
-  const int ai = lane & 31, ak = lane >> 5;
-  const int chunks = 1 << (k - 10);
-  for (int o = 0; o < P->nmop; ++o) {
-    const DOp op = dop[P->mop0 + o];
-    const float* U = ct + (-op.moff - 1);
-    // 5 tile-local gate bits, then the swizzled tile offsets of the 32 rows
-    const int32_t* ub = ukbits + op.mem0;
-    uint32_t bm = 0;
-    for (int i = 0; i < 5; ++i) bm |= 1u << ub[i];
-    const int32_t* rows = ub + 5;
-    float ur[16], ui[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int kk = 2 * q + ak;
-      ur[q] = U[2 * (ai * 32 + kk)];
-      ui[q] = U[2 * (ai * 32 + kk) + 1];
-    }
-    for (int chunk = wave; chunk < chunks; chunk += nwaves) {
-      // column index -> the tile bits outside the block, ascending; the
-      // swizzle is linear over XOR, so swz(cb | row) = swz(cb) ^ swz(row)
-      const uint32_t col = (uint32_t)chunk * 32 + ai;
-      uint32_t cb = 0;
-      for (int t = 0, j = 0; t < k; ++t)
-        if (!((bm >> t) & 1)) {
-          if ((col >> j) & 1) cb |= 1u << t;
-          ++j;
-        }
-      const uint32_t xc = swz(cb);
-      float2 x[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) x[q] = tile[xc ^ (uint32_t)(ak ? rows[2 * q + 1] : rows[2 * q])];
-      tcm_v16f dr = {0}, di = {0};
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        dr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].x, dr, 0, 0, 0);
-        dr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[q], x[q].y, dr, 0, 0, 0);
-        di = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].y, di, 0, 0, 0);
-        di = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[q], x[q].x, di, 0, 0, 0);
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = 8 * (r >> 2) + (r & 3);
-        tile[xc ^ (uint32_t)(ak ? rows[row + 4] : rows[row])] = make_float2(dr[r], di[r]);
-      }
-    }
-    __syncthreads();
+  // Every table read happens here, before the first store: reads inside the
+  // tile loop would be vector loads queued behind the prefetch and the stores.
+  // A operands: lane l of K-step q supplies U[l % 32][2q + l / 32]
+  for (int idx = tid; idx < nmop * 1024; idx += NTHR) {
+    const int o = idx >> 10, q = (idx >> 6) & 15, ln = idx & 63;
+    const float* U = ct + (-ops[o].moff - 1);
+    const int e = 2 * ((ln & 31) * 32 + 2 * q + (ln >> 5));
+    amat[idx] = make_float2(U[e], U[e + 1]);
   }
-  for (int it = 0; it < per; ++it) s[tile_dep(ro, L, it * nthr) + dlo] = tile[swz(it * nthr) ^ slo];
+  if (tid < nmop * 8) {
+    const int32_t* ub = ukbits + ops[tid >> 3].mem0;
+    const int i = tid & 7;
+    ebits[tid] = i < 5 ? swz(1u << ub[i]) : 0u;
+  }
+  // column c of a block -> the tile bits outside the block, ascending
+  for (int idx = tid; idx < nmop * NCOL; idx += NTHR) {
+    const int32_t* ub = ukbits + ops[idx / NCOL].mem0;
+    uint32_t bm = 0;
+    for (int j = 0; j < 5; ++j) bm |= 1u << ub[j];
+    const uint32_t col = (uint32_t)(idx % NCOL);
+    uint32_t cb = 0;
+    for (int b = 0, j = 0; b < K; ++b)
+      if (!((bm >> b) & 1)) {
+        if ((col >> j) & 1) cb |= 1u << b;
+        ++j;
+      }
+    xcol[idx] = swz(cb);
+  }
+  uint64_t bmask = 0;
+  for (int i = 0; i < nbl; ++i) bmask |= 1ull << P->blk_bits[i];
+  // pair it of this thread: tile elements x = 2 (it * NTHR + tid) and x + 1,
+  // adjacent in the state (the low L >= 4 tile bits are contiguous) and in
+  // LDS, at slots swz(x) and swz(x) ^ 1: in reverse order when swz(x) is odd
+  // (the swizzle folds bits 4, 8, 12 into bit 0)
+  uint32_t roff[PER2];
+  unrolled<PER2>([&](auto it) { roff[it] = tile_dep(ro, L, it * 2 * NTHR); });
+  const uint32_t dlo = tile_dep(ro, L, 2 * tid);
+  const uint32_t slo = swz(2 * tid);
+  tcm_v4f* tile4 = reinterpret_cast<tcm_v4f*>(smem);
+  auto put4 = [&](int it, tcm_v4f v) {
+    const uint32_t i = swz(it * 2 * NTHR) ^ slo;
+    if (i & 1) v = v.zwxy;
+    tile4[i >> 1] = v;
+  };
+  auto get4 = [&](int it) -> tcm_v4f {
+    const uint32_t i = swz(it * 2 * NTHR) ^ slo;
+    tcm_v4f v = tile4[i >> 1];
+    if (i & 1) v = v.zwxy;
+    return v;
+  };
+  // tile tt: batch item tt >> nbl; its block bits (ascending) hold the low nbl bits of tt
+  auto tile_ptr = [&](int64_t tt) {
+    const uint64_t blk = (uint64_t)tt & ((1ull << nbl) - 1);
+    uint64_t m = bmask, base = 0;
+    for (int i = 0; i < nbl; ++i) {
+      const uint64_t low = m & (~m + 1);
+      if ((blk >> i) & 1) base |= low;
+      m ^= low;
+    }
+    return st + ((uint64_t)tt >> nbl) * sstride + base;
+  };
+  auto glb4 = [&](float2* sp, int it) -> tcm_v4f* { return reinterpret_cast<tcm_v4f*>(sp + roff[it] + dlo); };
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ai = lane & 31, ak = lane >> 5;
+  // the pass's blocks on the tile in LDS, each followed by a barrier
+  auto blocks = [&]() {
+    for (int o = 0; o < nmop; ++o) {
+      if (wave < CHUNKS) {
+        const uint32_t* ob = ebits + o * 8;
+        uint32_t e[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) e[i] = __builtin_amdgcn_readfirstlane(ob[i]);
+        const uint32_t xc = xcol[o * NCOL + wave * 32 + ai];
+        // read rows 2q + ak; written rows 8 (r / 4) + 4 ak + r % 4
+        const uint32_t rb = xc ^ (ak ? e[4] : 0u);
+        const uint32_t wb = xc ^ (ak ? e[2] : 0u);
+        const float2* A = amat + o * 1024 + lane;
+        float2 a[16], x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          a[q] = A[q * 64];
+          const uint32_t g = ((q & 1) ? e[3] : 0u) ^ ((q & 2) ? e[2] : 0u) ^ ((q & 4) ? e[1] : 0u) ^
+                             ((q & 8) ? e[0] : 0u);
+          x[q] = tile[rb ^ g];
+        }
+        tcm_v16f p1 = {0}, p2 = {0}, p3 = {0};
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x, x[q].x, p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].y, x[q].y, p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x + a[q].y, x[q].x + x[q].y, p3, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t h = ((r & 1) ? e[4] : 0u) ^ ((r & 2) ? e[3] : 0u) ^ ((r & 4) ? e[1] : 0u) ^
+                             ((r & 8) ? e[0] : 0u);
+          tile[wb ^ h] = make_float2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
+        }
+      }
+      __syncthreads();
+    }
+  };
+  float2* s = tile_ptr(t);
+  tcm_v4f pre[PER2];
+  unrolled<PER2>([&](auto it) { pre[it] = *glb4(s, it); });
+  unrolled<PER2>([&](auto it) { put4(it, pre[it]); });
+  // every tile but the last: the next tile's loads are issued before this
+  // tile's matrix-core work; each thread then stores the LDS slots it wrote
+  // and refills them with the next tile (same slots: no barrier between)
+  for (;;) {
+    const int64_t tn = t + gridDim.x;
+    if (tn >= ntiles) break;
+    float2* sn = tile_ptr(tn);
+    unrolled<PER2>([&](auto it) { pre[it] = *glb4(sn, it); });
+    __syncthreads();  // the tile (and, first time, the staged operands) in LDS
+    blocks();
+    unrolled<PER2>([&](auto it) {
+      *glb4(s, it) = get4(it);
+      put4(it, pre[it]);
+    });
+    t = tn;
+    s = sn;
+  }
+  __syncthreads();
+  blocks();
+  unrolled<PER2>([&](auto it) { *glb4(s, it) = get4(it); });
 }
 
-void launch_mfma_tile(const DPass& P, const DPass* passes_dev, int pi, const DOp* dop_dev, const int32_t* ukbits_dev,
+// Blocks of `kern` (512 threads, `lds` bytes of LDS) resident on the whole
+// device at once: the grid of a persistent kernel.
+template <typename Kern>
+static int64_t resident_grid(Kern kern, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, size_t>, int64_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(dev, (const void*)kern, lds);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 512, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) ncu = 256;
+  return cache[key] = (int64_t)per_cu * ncu;
+}
+
+template <typename K>
+static void ensure_lds_attr(K kern, size_t bytes);
+
+template <int K>
+static void launch_mfma_tile_k(const DPass& P, const DPass* passes_dev, int pi, const DOp* dop_dev,
+                               const int32_t* ukbits_dev, const uint32_t* runoff_dev, const void* ctab, void* state,
+                               int n, int64_t batch, hipStream_t st) {
+  const size_t lds = ((size_t)8 << K) + (size_t)P.nmop * (8192 + 32 + (4 << (K - 5)));
+  ensure_lds_attr(k_mfma_tile<K>, lds);
+  const int64_t ntiles = batch << P.nblk_log;
+  const int64_t grid = std::min<int64_t>(ntiles, resident_grid(k_mfma_tile<K>, lds));
+  hipLaunchKernelGGL(k_mfma_tile<K>, dim3((unsigned)grid), dim3(512), lds, st, passes_dev, pi, dop_dev, ukbits_dev,
+                     runoff_dev, (const float*)ctab, (float2*)state, (int64_t)1 << n, ntiles);
+}
+
+bool launch_mfma_tile(const DPass& P, const DPass* passes_dev, int pi, const DOp* dop_dev, const int32_t* ukbits_dev,
                       const uint32_t* runoff_dev, const void* ctab, void* state, int n, int64_t batch,
                       hipStream_t st) {
-  const int64_t blocks = batch << P.nblk_log;
-  hipLaunchKernelGGL(k_mfma_tile, dim3((unsigned)blocks), dim3(512), (size_t)8 << P.k, st, passes_dev, pi, dop_dev,
-                     ukbits_dev, runoff_dev, (const float*)ctab, (float2*)state, (int64_t)1 << n);
+  if (P.nmop < 1 || P.nmop > kMfmaTileMaxOps || P.nblk_log != n - P.k) return false;
+  switch (P.k) {
+    case 10: launch_mfma_tile_k<10>(P, passes_dev, pi, dop_dev, ukbits_dev, runoff_dev, ctab, state, n, batch, st); return true;
+    case 11: launch_mfma_tile_k<11>(P, passes_dev, pi, dop_dev, ukbits_dev, runoff_dev, ctab, state, n, batch, st); return true;
+    case 12: launch_mfma_tile_k<12>(P, passes_dev, pi, dop_dev, ukbits_dev, runoff_dev, ctab, state, n, batch, st); return true;
+    case 13: launch_mfma_tile_k<13>(P, passes_dev, pi, dop_dev, ukbits_dev, runoff_dev, ctab, state, n, batch, st); return true;
+    default: return false;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -840,7 +840,6 @@ static void mfma_tile_run(const Program& P, Sched& S, const std::vector<int>& it
   for (size_t ci = 0; ci < clusters.size(); ++ci) {
     HPass hp;
     hp.mfma = true;
-    hp.gates = clusters[ci];
     uint64_t tmask = lmask | used[ci];
     for (int b = 0; b < n && popc64(tmask) < kM; ++b) tmask |= (1ull << b);
     for (int b = 0; b < n; ++b)
@@ -848,7 +847,12 @@ static void mfma_tile_run(const Program& P, Sched& S, const std::vector<int>& it
     int L = 0;
     while (L < (int)hp.tile.size() && hp.tile[L] == L) ++L;
     hp.L = L;
-    S.passes.push_back(hp);
+    // at most kMfmaTileMaxOps blocks per pass (their matrices share the LDS)
+    const std::vector<int>& c = clusters[ci];
+    for (size_t g0 = 0; g0 < c.size(); g0 += kMfmaTileMaxOps) {
+      hp.gates.assign(c.begin() + g0, c.begin() + std::min(c.size(), g0 + (size_t)kMfmaTileMaxOps));
+      S.passes.push_back(hp);
+    }
   }
 }
 
@@ -940,13 +944,6 @@ static void sched_fused(const Program& P, Sched& S, int k, int R, int Lwant, boo
         op.mem0 = (int)S.ukbits.size();
         op.flags = OP_F_MFMA;
         for (int i = 0; i < 5; ++i) S.ukbits.push_back(local_of_bit[g.bits[i]]);
-        // then the swizzled tile offsets of the 32 gate rows
-        for (int r = 0; r < 32; ++r) {
-          uint32_t x = 0;
-          for (int i = 0; i < 5; ++i)
-            if ((r >> (4 - i)) & 1) x |= 1u << local_of_bit[g.bits[i]];
-          S.ukbits.push_back((int32_t)swz_host(x));
-        }
         S.dop.push_back(op);
       }
       dp.nmop = (int)hp.gates.size();

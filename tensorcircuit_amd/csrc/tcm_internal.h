@@ -87,6 +87,10 @@ struct DGroup {
   int32_t ukbits[5]; // tile-local bits of that op (first = most significant)
 };
 
+// dense 5-qubit blocks per matrix-core tile pass (their A operands share the
+// LDS with the tile: 8 KiB each)
+constexpr int kMfmaTileMaxOps = 8;
+
 struct DPass {
   int32_t k;          // tile bits
   int32_t L;          // contiguous low bits inside the tile (run length 2^L)

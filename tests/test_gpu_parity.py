@@ -604,6 +604,44 @@ def test_brickwork_fused_mfma_vs_c128(gpu):
             np.testing.assert_allclose(np_of(s32).reshape(-1), ref, atol=2e-6)
 
 
+def test_mfma_tile_passes_every_width(gpu):
+    """Matrix-core tile passes at every tile width (10..13 bits) on a 20-qubit
+    batch of 3 states: more tiles than resident blocks, so every block runs
+    several tiles and loads the next one during the matrix-core work.  Twelve
+    blocks on 5-subsets of 9 qubits share one 13-bit tile and are split into
+    passes of 8 and 4; two more sit on the low qubits.  The result equals the
+    complex128 VALU run of the same gates (the c128 path is pinned against the
+    oracle above)."""
+    import itertools
+
+    from tensorcircuit_amd import _lib, engine
+    from tensorcircuit_amd.engine import GateSpec
+
+    n, batch = 20, 3
+    rng = np.random.default_rng(20)
+    subsets = list(itertools.combinations(range(9), 5))
+    qsets = [tuple(int(q) for q in rng.permutation(subsets[i])) for i in rng.choice(len(subsets), 12, replace=False)]
+    qsets += [tuple(range(n - 5, n)), (n - 1, 3, n - 7, 12, n - 2)]
+    specs = []
+    for qs in qsets:
+        u = np.linalg.qr(rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32)))[0]
+        specs.append(GateSpec(_lib.G_MAT, qs, -1, 0.0, 1.0, u))
+    s0 = rng.standard_normal((batch, 2**n)) + 1j * rng.standard_normal((batch, 2**n))
+    s0 /= np.linalg.norm(s0, axis=1, keepdims=True)
+    p64 = engine.Program(n, specs, 0, fuse=1, mfma=0)
+    ref = np_of(p64.forward(None, batch, "complex128", inputs=torch.as_tensor(s0))[0])
+    for tb in (13, 12, 11, 10):
+        pt = engine.Program(n, specs, 0, fuse=1, mfma=2, tile_bits=tb)
+        passes = pt.describe("complex64", 0)["sched"]["passes"]
+        assert all(p["mfma"] == 1 and len(p["tile"]) == tb for p in passes)
+        assert max(len(p["gates"]) for p in passes) <= 8
+        assert sorted(g for p in passes for g in p["gates"]) == list(range(len(specs)))
+        if tb == 13:
+            assert [len(p["gates"]) for p in passes][:2] == [8, 4]
+        st, _ = pt.forward(None, batch, "complex64", inputs=torch.as_tensor(s0).to(torch.complex64))
+        np.testing.assert_allclose(np_of(st), ref, atol=3e-6, err_msg="tile bits %d" % tb)
+
+
 # ------------------------------------------------------ batched uneven (C3) ---
 
 def test_forward_multi_uneven(gpu):
