@@ -64,9 +64,9 @@ def parse():
     ap.add_argument("--config", choices=["c4", "c2", "c3", "c5"], default="c4",
                     help="c4 (default): 24q adjoint energy+grad; c2: 26q forward + energy; "
                          "c5: 30q fused-MFMA brickwork with a complex128 check")
-    ap.add_argument("--kernel-only", choices=["fwd", "adj", "copy"], default=None,
-                    help="run only the fused forward passes, the adjoint sweep, or a state copy of the same "
-                         "size (for counter profiles); prints no JSON")
+    ap.add_argument("--kernel-only", choices=["fwd", "adj", "copy", "sweep"], default=None,
+                    help="run only the fused forward passes, the adjoint sweep, a state copy of the same "
+                         "size, or the 1-qubit gate sweeps (for counter profiles); prints no JSON")
     return ap.parse_args()
 
 
@@ -118,8 +118,12 @@ def main():
     params = torch.as_tensor(rng.uniform(0, 2 * np.pi, size=(B, P)).astype(np.float32)).to(dev)
     if args.kernel_only:
         st, ck = prog.forward(params, B, "complex64", want_ckpts=True)
+        if args.kernel_only == "sweep":
+            sprog, sparams = _sweep_program(n, B)
         for _ in range(max(args.steps, 1)):
-            if args.kernel_only == "fwd":
+            if args.kernel_only == "sweep":
+                sprog.forward(sparams, B, "complex64", out=st)
+            elif args.kernel_only == "fwd":
                 prog.forward(params, B, "complex64", out=st)
             elif args.kernel_only == "adj":
                 prog.vjp(params, ck, st, st, B, "complex64", False)
@@ -240,6 +244,32 @@ def main():
                                     "fwd_frac": round(vf / VALU_PEAK_WAVE_INSTS, 4)}
         print("fused fwd pass %.3f ms (%.0f GB/s), adjoint pass %.3f ms (%.0f GB/s)"
               % (fwd_launch_ms, fwd_gbs, adj_launch_ms, adj_gbs), file=sys.stderr)
+        # gate sweeps: one 1-qubit gate per launch (ry then rz on every
+        # qubit) over the same 4 GiB batch, next to a device copy of it
+        sprog, sparams = _sweep_program(n, B)
+        sprog.forward(sparams, B, "complex64", out=st)
+        cp = torch.empty_like(st)
+        ev[0].record(stream)
+        for _ in range(reps):
+            sprog.forward(sparams, B, "complex64", out=st)
+        ev[1].record(stream)
+        for _ in range(reps):
+            cp.copy_(st)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        sweep_launch_ms = ev[0].elapsed_time(ev[1]) / reps / sprog.npasses
+        copy_ms = ev[1].elapsed_time(ev[2]) / reps
+        sweep_gbs = fwd_bytes / (sweep_launch_ms * 1e-3) / 1e9
+        copy_gbs = fwd_bytes / (copy_ms * 1e-3) / 1e9
+        del cp
+        out["roofline_sweep"] = {"bound": "hbm", "kernel": "k_plain1 (one 1-qubit gate per launch)",
+                                 "achieved": round(sweep_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(sweep_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": fwd_bytes,
+                                 "avg_launch_ms": round(sweep_launch_ms, 4), "launches": sprog.npasses,
+                                 "device_copy_gbs": round(copy_gbs, 1),
+                                 "frac_of_copy": round(sweep_gbs / copy_gbs, 4)}
+        print("gate sweep %.3f ms per 1-qubit gate (%.0f GB/s, %.3f of peak), device copy %.0f GB/s"
+              % (sweep_launch_ms, sweep_gbs, sweep_gbs / HBM_PEAK_GBS, copy_gbs), file=sys.stderr)
 
     # ---------------------------------------------------------- CPU baseline
     if not args.no_cpu_baseline:
@@ -263,6 +293,25 @@ def main():
               file=sys.stderr)
 
     print(json.dumps(out))
+
+
+def _sweep_program(n, B):
+    """ry then rz on every qubit, one 1-qubit gate per launch (no fusion),
+    for a batch of B states."""
+    import torch
+
+    import tensorcircuit_amd as tc
+    from tensorcircuit_amd import engine
+
+    sc = tc.Circuit(n)
+    for i in range(n):
+        sc.ry(i, theta=0.3 + 0.01 * i)
+    for i in range(n):
+        sc.rz(i, theta=0.7 - 0.01 * i)
+    sspecs, sth = sc._lower()
+    sprog = engine.Program(n, sspecs, len(sth), fuse=0)
+    sparams = torch.tensor([float(x) for x in sth], dtype=torch.float32, device=engine.device()).repeat(B, 1)
+    return sprog, sparams
 
 
 def _timed(fn, steps, warmup):
