@@ -1038,8 +1038,10 @@ __global__ void __launch_bounds__(TB) k_fused_adj(const DPass* __restrict__ pass
 // dense 5-qubit gate on FP32 matrix cores.  The state is viewed as a
 // 32 x (2^(n-5)) matrix X (rows = the 5 gate bits, row MSB = gate qubit 0,
 // columns = all other bits) and Y = U X is computed per 32-column chunk by
-// one wave with v_mfma_f32_32x32x2f32 (K = 2 per instruction):
-//   Yr = Ur Xr - Ui Xi,  Yi = Ur Xi + Ui Xr.
+// one wave with v_mfma_f32_32x32x2f32 (K = 2 per instruction), as three
+// real products (Gauss's complex multiplication):
+//   P1 = Ur Xr,  P2 = Ui Xi,  P3 = (Ur + Ui)(Xr + Xi),
+//   Yr = P1 - P2,  Yi = P3 - P1 - P2.
 // Operand layouts (CDNA3/4): lane l supplies A[l % 32][l / 32] and
 // B[l / 32][l % 32]; accumulator register r of lane l holds
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
@@ -1110,18 +1112,17 @@ __global__ void __launch_bounds__(256) k_mfma_u5(DOp op, const int32_t* __restri
 #pragma unroll
       for (int q = 0; q < 16; ++q) xn[q] = sn[rowoff(2 * q + ak)];
     }
-    tcm_v16f dr = {0}, di = {0};
+    tcm_v16f p1 = {0}, p2 = {0}, p3 = {0};
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      dr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].x, dr, 0, 0, 0);
-      dr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[q], x[q].y, dr, 0, 0, 0);
-      di = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].y, di, 0, 0, 0);
-      di = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[q], x[q].x, di, 0, 0, 0);
+      p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q], x[q].x, p1, 0, 0, 0);
+      p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[q], x[q].y, p2, 0, 0, 0);
+      p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[q] + ui[q], x[q].x + x[q].y, p3, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r >> 2) + 4 * ak + (r & 3);
-      s[rowoff(row)] = make_float2(dr[r], di[r]);
+      s[rowoff(row)] = make_float2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
     }
     if (more) {
 #pragma unroll
