@@ -448,7 +448,7 @@ def run_c3(args):
             c = tc.Circuit(n)
             hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
             specs, thetas = c._lower()
-            prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1)
+            prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1, reg_slots=args.reg_slots)
             prog.compile("complex64", 0)
             items.append((prog, pa, st))
         t, _ = _timed(make_seq_step(items), 3, 1)
@@ -504,7 +504,7 @@ def run_c2(args):
     ham = engine.PauliSum(n, codes, w)
     p = torch.as_tensor(workloads.hea_params(1002, L, n).reshape(1, -1)).cuda()
     t_stage = time.perf_counter()
-    prog = engine.Program(n, specs, len(thetas), fuse=1, jit=args.jit)
+    prog = engine.Program(n, specs, len(thetas), fuse=1, jit=args.jit, reg_slots=args.reg_slots)
     if args.jit:
         prog.compile("complex64", 0)
         prog.compile("complex128", 0)

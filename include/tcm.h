@@ -88,7 +88,8 @@ typedef struct tcm_opts {
   int32_t adj_tile_bits; /* adjoint tile size log2; 0 = auto */
   int32_t ckpt_every;  /* adjoint: checkpoint the forward state every
                           `ckpt_every` barriers (0 = none) */
-  int32_t reg_slots;   /* register-resident qubits per thread; 0 = auto (4) */
+  int32_t reg_slots;   /* register-resident qubits per thread; 0 = auto (5 for
+                          compiled complex64 passes on >= 20 qubits, else 4) */
   int32_t mfma;        /* 1: dense 5-qubit gates run on FP32 matrix cores,
                           one streaming pass per gate; 2: consecutive such
                           gates grouped into LDS tile passes of 2^k
