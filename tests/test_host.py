@@ -372,3 +372,29 @@ def test_mfma_tile_pass_schedule():
     sched = p2.describe("complex64", 0)["sched"]
     assert all(p["mfma"] == 1 and len(p["tile"]) == 13 for p in sched["passes"])
     assert sorted(g for p in sched["passes"] for g in p["gates"]) == list(range(76))
+
+
+def test_default_register_slots():
+    """Compiled complex64 passes hold 5 register-resident qubits per thread on
+    states of >= 20 qubits and 4 below (a 10-qubit HEA stays one fused pass
+    instead of falling back to one launch per gate); complex128 and the
+    generic kernels keep 4; an explicit reg_slots wins."""
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    def prog(n, L, **kw):
+        c = Circuit(n)
+        hea_layers(c, np.zeros((2 * L, n), np.float32), L, barrier=False)
+        specs, thetas = c._lower()
+        return engine.Program(n, specs, len(thetas), fuse=1, **kw)
+
+    p10 = prog(10, 6, jit=1)
+    assert p10.describe("complex64", 0)["sched"]["R"] == 4
+    assert p10.npasses == 1
+    p16 = prog(16, 2, jit=1)
+    assert p16.describe("complex64", 0)["sched"]["R"] == 4
+    p20 = prog(20, 2, jit=1)
+    assert p20.describe("complex64", 0)["sched"]["R"] == 5
+    assert p20.describe("complex128", 0)["sched"]["R"] == 4
+    assert prog(20, 2, jit=0).describe("complex64", 0)["sched"]["R"] == 4
+    assert prog(16, 2, jit=1, reg_slots=5).describe("complex64", 0)["sched"]["R"] == 5
