@@ -95,7 +95,7 @@ typedef struct tcm_opts {
                           gates grouped into LDS tile passes of 2^k
                           amplitudes (k = tile_bits clamped to 10..13,
                           default 13; at most 8 gates per pass; fewer HBM
-                          passes, 1.5x faster than 1 on a 30-qubit depth-20
+                          passes, 1.6x faster than 1 on a 30-qubit depth-20
                           brickwork); complex64 only, complex128 keeps the
                           VALU kernel */
   int32_t jit;         /* 1: compile every tile pass into its own kernel

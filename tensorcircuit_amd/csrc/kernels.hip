@@ -1244,6 +1244,7 @@ __global__ void __launch_bounds__(512) k_mfma_tile(const DPass* __restrict__ pas
   auto glb4 = [&](float2* sp, int it) -> tcm_v4f* { return reinterpret_cast<tcm_v4f*>(sp + roff[it] + dlo); };
   const int lane = tid & 63, wave = tid >> 6;
   const int ai = lane & 31, ak = lane >> 5;
+  float2 a0[16];  // A operands of the pass's first block (filled once staged)
   // the pass's blocks on the tile in LDS, each followed by a barrier
   auto blocks = [&]() {
     for (int o = 0; o < nmop; ++o) {
@@ -1260,7 +1261,7 @@ __global__ void __launch_bounds__(512) k_mfma_tile(const DPass* __restrict__ pas
         float2 a[16], x[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-          a[q] = A[q * 64];
+          a[q] = o == 0 ? a0[q] : A[q * 64];
           const uint32_t g = ((q & 1) ? e[3] : 0u) ^ ((q & 2) ? e[2] : 0u) ^ ((q & 4) ? e[1] : 0u) ^
                              ((q & 8) ? e[0] : 0u);
           x[q] = tile[rb ^ g];
@@ -1286,6 +1287,9 @@ __global__ void __launch_bounds__(512) k_mfma_tile(const DPass* __restrict__ pas
   tcm_v4f pre[PER2];
   unrolled<PER2>([&](auto it) { pre[it] = *glb4(s, it); });
   unrolled<PER2>([&](auto it) { put4(it, pre[it]); });
+  // the first block's A operands stay in registers for every tile
+  __syncthreads();
+  unrolled<16>([&](auto q) { a0[q] = amat[q * 64 + lane]; });
   // every tile but the last: the next tile's loads are issued before this
   // tile's matrix-core work; each thread then stores the LDS slots it wrote
   // and refills them with the next tile (same slots: no barrier between)
