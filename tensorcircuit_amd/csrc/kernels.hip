@@ -1244,52 +1244,59 @@ __global__ void __launch_bounds__(512) k_mfma_tile(const DPass* __restrict__ pas
   auto glb4 = [&](float2* sp, int it) -> tcm_v4f* { return reinterpret_cast<tcm_v4f*>(sp + roff[it] + dlo); };
   const int lane = tid & 63, wave = tid >> 6;
   const int ai = lane & 31, ak = lane >> 5;
-  float2 a0[16];  // A operands of the pass's first block (filled once staged)
-  // the pass's blocks on the tile in LDS, each followed by a barrier
-  auto blocks = [&]() {
-    for (int o = 0; o < nmop; ++o) {
-      if (wave < CHUNKS) {
-        const uint32_t* ob = ebits + o * 8;
-        uint32_t e[5];
+  // A operands of the pass's first two blocks (filled once staged)
+  float2 a0[16], a1[16];
+  // block o of the pass on the tile in LDS; geta(q) gives its A operand of K-step q
+  auto block = [&](int o, auto geta) {
+    if (wave < CHUNKS) {
+      const uint32_t* ob = ebits + o * 8;
+      uint32_t e[5];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) e[i] = __builtin_amdgcn_readfirstlane(ob[i]);
-        const uint32_t xc = xcol[o * NCOL + wave * 32 + ai];
-        // read rows 2q + ak; written rows 8 (r / 4) + 4 ak + r % 4
-        const uint32_t rb = xc ^ (ak ? e[4] : 0u);
-        const uint32_t wb = xc ^ (ak ? e[2] : 0u);
-        const float2* A = amat + o * 1024 + lane;
-        float2 a[16], x[16];
+      for (int i = 0; i < 5; ++i) e[i] = __builtin_amdgcn_readfirstlane(ob[i]);
+      const uint32_t xc = xcol[o * NCOL + wave * 32 + ai];
+      // read rows 2q + ak; written rows 8 (r / 4) + 4 ak + r % 4
+      const uint32_t rb = xc ^ (ak ? e[4] : 0u);
+      const uint32_t wb = xc ^ (ak ? e[2] : 0u);
+      float2 a[16], x[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          a[q] = o == 0 ? a0[q] : A[q * 64];
-          const uint32_t g = ((q & 1) ? e[3] : 0u) ^ ((q & 2) ? e[2] : 0u) ^ ((q & 4) ? e[1] : 0u) ^
-                             ((q & 8) ? e[0] : 0u);
-          x[q] = tile[rb ^ g];
-        }
-        tcm_v16f p1 = {0}, p2 = {0}, p3 = {0};
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x, x[q].x, p1, 0, 0, 0);
-          p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].y, x[q].y, p2, 0, 0, 0);
-          p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x + a[q].y, x[q].x + x[q].y, p3, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const uint32_t h = ((r & 1) ? e[4] : 0u) ^ ((r & 2) ? e[3] : 0u) ^ ((r & 4) ? e[1] : 0u) ^
-                             ((r & 8) ? e[0] : 0u);
-          tile[wb ^ h] = make_float2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
-        }
+      for (int q = 0; q < 16; ++q) {
+        a[q] = geta(q);
+        const uint32_t g = ((q & 1) ? e[3] : 0u) ^ ((q & 2) ? e[2] : 0u) ^ ((q & 4) ? e[1] : 0u) ^
+                           ((q & 8) ? e[0] : 0u);
+        x[q] = tile[rb ^ g];
       }
-      __syncthreads();
+      tcm_v16f p1 = {0}, p2 = {0}, p3 = {0};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x, x[q].x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].y, x[q].y, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x + a[q].y, x[q].x + x[q].y, p3, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t h = ((r & 1) ? e[4] : 0u) ^ ((r & 2) ? e[3] : 0u) ^ ((r & 4) ? e[1] : 0u) ^
+                           ((r & 8) ? e[0] : 0u);
+        tile[wb ^ h] = make_float2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
+      }
     }
+    __syncthreads();
+  };
+  // the pass's blocks in order, each followed by a barrier
+  auto blocks = [&]() {
+    block(0, [&](int q) { return a0[q]; });
+    if (nmop > 1) block(1, [&](int q) { return a1[q]; });
+    for (int o = 2; o < nmop; ++o) block(o, [&](int q) { return amat[o * 1024 + q * 64 + lane]; });
   };
   float2* s = tile_ptr(t);
   tcm_v4f pre[PER2];
   unrolled<PER2>([&](auto it) { pre[it] = *glb4(s, it); });
   unrolled<PER2>([&](auto it) { put4(it, pre[it]); });
-  // the first block's A operands stay in registers for every tile
+  // the first two blocks' A operands stay in registers for every tile
   __syncthreads();
-  unrolled<16>([&](auto q) { a0[q] = amat[q * 64 + lane]; });
+  unrolled<16>([&](auto q) {
+    a0[q] = amat[q * 64 + lane];
+    a1[q] = amat[(nmop > 1 ? 1024 : 0) + q * 64 + lane];
+  });
   // every tile but the last: the next tile's loads are issued before this
   // tile's matrix-core work; each thread then stores the LDS slots it wrote
   // and refills them with the next tile (same slots: no barrier between)
