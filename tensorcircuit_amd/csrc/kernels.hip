@@ -566,25 +566,30 @@ __global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, const DFactor*
     T* o = mt + r.out;
     switch (r.kind) {
       case TCM_G_RX: {
-        double c = cos(0.5 * a), s = sin(0.5 * a);
+        double c, s;
+        sincos(0.5 * a, &s, &c);
         o[0] = (T)c; o[1] = 0; o[2] = 0; o[3] = (T)-s; o[4] = 0; o[5] = (T)-s; o[6] = (T)c; o[7] = 0;
       } break;
       case TCM_G_RY: {
-        double c = cos(0.5 * a), s = sin(0.5 * a);
+        double c, s;
+        sincos(0.5 * a, &s, &c);
         o[0] = (T)c; o[1] = (T)-s; o[2] = (T)s; o[3] = (T)c;
       } break;
       case TCM_G_RZ: {
-        double c = cos(0.5 * a), s = sin(0.5 * a);
+        double c, s;
+        sincos(0.5 * a, &s, &c);
         o[0] = (T)c; o[1] = (T)-s; o[2] = (T)c; o[3] = (T)s;
       } break;
       case TCM_G_RZZ: {
-        double c = cos(0.5 * a), s = sin(0.5 * a);
+        double c, s;
+        sincos(0.5 * a, &s, &c);
         o[0] = (T)c; o[1] = (T)-s; o[2] = (T)c; o[3] = (T)s;
         o[4] = (T)c; o[5] = (T)s; o[6] = (T)c; o[7] = (T)-s;
       } break;
       case TCM_G_RXX:
       case TCM_G_RYY: {
-        double c = cos(0.5 * a), s = sin(0.5 * a);
+        double c, s;
+        sincos(0.5 * a, &s, &c);
         for (int q = 0; q < 32; ++q) o[q] = 0;
         for (int d = 0; d < 4; ++d) o[2 * (d * 4 + d)] = (T)c;
         if (r.kind == TCM_G_RXX) {
@@ -599,11 +604,14 @@ __global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, const DFactor*
         }
       } break;
       case TCM_G_PHASE: {
-        o[0] = 1; o[1] = 0; o[2] = (T)cos(a); o[3] = (T)sin(a);
+        double c, s;
+        sincos(a, &s, &c);
+        o[0] = 1; o[1] = 0; o[2] = (T)c; o[3] = (T)s;
       } break;
       case TCM_G_EXP1: {
         // cos(a) I - i sin(a) G, G from the constant table (full matrix)
-        double c = cos(a), s = sin(a);
+        double c, s;
+        sincos(a, &s, &c);
         const T* g = ct + r.goff;
         const int d = (r.layout == OK_U1 || r.layout == OK_D1) ? 2 : 4;
         if (r.layout == OK_D1 || r.layout == OK_D2) {
