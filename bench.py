@@ -226,7 +226,11 @@ def main():
         kname = "tcm_adj_k* (compiled adjoint passes)" if args.jit else "k_fused_adj"
         out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": round(adj_gbs, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
-                           "traffic": traffic, "bytes_per_launch": adj_bytes,
+                           "traffic": traffic,
+                           "traffic_source": None if traffic is None else
+                           "recorded constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over this workload's "
+                           "adjoint pass dispatches (profiles/tcm_adj_passes_hea24_b32.md)",
+                           "bytes_per_launch": adj_bytes,
                            "avg_launch_ms": round(adj_launch_ms, 4), "launches_per_step": n_adj}
         out["roofline_fwd"] = {"kernel": "tcm_fwd_k* (compiled forward passes)" if args.jit else "k_fused_fwd",
                                "achieved": round(fwd_gbs, 1), "unit": "GB/s",
@@ -271,28 +275,66 @@ def main():
         print("gate sweep %.3f ms per 1-qubit gate (%.0f GB/s, %.3f of peak), device copy %.0f GB/s"
               % (sweep_launch_ms, sweep_gbs, sweep_gbs / HBM_PEAK_GBS, copy_gbs), file=sys.stderr)
 
+    # ------------------------------------------- precision against complex128
+    # the timed complex64 path against the complex128 run of the same program
+    # on the first parameter sets (the same float32 angles, widened)
+    nchk = min(2, B)
+    prog128 = engine.Program(n, specs, P, fuse=1, ckpt_every=args.ckpt, tile_bits=args.tile,
+                             adj_tile_bits=args.adj_tile, jit=args.jit, greedy=args.greedy)
+    e128, g128 = prog128.energy_grad(params[:nchk].double().contiguous(), ham, nchk, "complex128")
+    torch.cuda.synchronize()
+    out["max_abs_dgrad_vs_c128"] = float((g[:nchk] - g128).abs().max())
+    out["max_rel_denergy_vs_c128"] = float(((e[:nchk] - e128).abs() / e128.abs()).max())
+    out["precision_check_sets"] = nchk
+    print("vs complex128: max|dgrad| %.2e, max rel|dE| %.2e" % (out["max_abs_dgrad_vs_c128"],
+                                                               out["max_rel_denergy_vs_c128"]), file=sys.stderr)
+    del prog128, e128, g128
+
     # ---------------------------------------------------------- CPU baseline
     if not args.no_cpu_baseline:
-        from oracle import plain_torch
-
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(int(threads), 16))
-        dt, _, _ = plain_torch.time_energy_grad(n, 1, seed=1004, threads=threads)
-        # one layer + the full energy; scale by gate applications per eval:
-        # 71 per layer, about 70 for the 47 expectation terms
-        per_layer = 3 * n - 1
-        energy_apps = 2 * (n - 1) + n
-        scale = (L * per_layer + energy_apps) / (per_layer + energy_apps)
-        cpu_eval_s = dt * scale
-        out["cpu_baseline"] = {"value": round(1.0 / cpu_eval_s, 6), "unit": "evals/s", "cores": threads,
-                               "kind": "port",
-                               "sample": "1 of %d layers of the %dq HEA + full TFIM energy, value_and_grad by "
-                                         "torch autograd (gate-by-gate tensordot, complex64), %.1f s, scaled by "
-                                         "gate applications x%.2f" % (L, n, dt, scale)}
-        print("cpu baseline: %.1f s per eval (sample %.1f s, %d threads)" % (cpu_eval_s, dt, threads),
-              file=sys.stderr)
+        out["cpu_baseline"] = _cpu_baseline(n, L)
 
     print(json.dumps(out))
+
+
+def _cpu_baseline(n, L):
+    """The reference's plain method (gate-by-gate tensordot, torch autograd,
+    complex64) on the host cores.  The 12-qubit 4-layer case is timed in full
+    (median of 5).  A 24-qubit 20-layer eval takes about a minute, so it is
+    sampled: energy + gradient at 1 and at 3 layers (each layer under
+    per-layer rematerialisation), twice each after a discarded warm-up;
+    20 layers = t(1) + 19 (t(3) - t(1)) / 2 per pair, and the two pairs give
+    the spread."""
+    import statistics
+
+    from oracle import plain_torch
+
+    # the box's CPU share for one GPU is OMP_NUM_THREADS; otherwise the affinity mask
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if threads <= 0:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    plain_torch.time_energy_grad(12, 1, seed=1001, threads=threads)  # warm-up
+    c1 = [plain_torch.time_energy_grad(12, 4, seed=1001, threads=threads)[0] for _ in range(5)]
+    c1_s = statistics.median(c1)
+    plain_torch.time_energy_grad(n, 1, seed=1004, threads=threads, remat=True)  # warm-up, discarded
+    samples, ests = [], []
+    for _ in range(2):
+        t1 = plain_torch.time_energy_grad(n, 1, seed=1004, threads=threads, remat=True)[0]
+        t3 = plain_torch.time_energy_grad(n, 3, seed=1004, threads=threads, remat=True)[0]
+        samples.append((round(t1, 3), round(t3, 3)))
+        ests.append(t1 + (L - 1) * max(t3 - t1, 0.0) / 2)
+    cpu_eval_s = sum(ests) / len(ests)
+    print("cpu baseline: C1 %.4f s/eval; %dq (t1, t3) %s -> %.1f s per %d-layer eval (%d threads)"
+          % (c1_s, n, samples, cpu_eval_s, L, threads), file=sys.stderr)
+    return {"value": round(1.0 / cpu_eval_s, 6), "unit": "evals/s", "cores": threads, "kind": "port",
+            "cpu_model": plain_torch.cpu_model(),
+            "sample": "%dq HEA + TFIM energy and gradient by torch autograd (gate-by-gate tensordot, complex64, "
+                      "each layer rematerialised), timed at 1 and 3 layers twice; %d layers = t(1) + %d (t(3) - "
+                      "t(1)) / 2" % (n, L, L - 1),
+            "samples_s_1_and_3_layers": samples,
+            "eval_s_estimates": [round(x, 2) for x in ests],
+            "c1_12q_4l_evals_per_s": round(1.0 / c1_s, 3),
+            "c1_samples_s": [round(x, 5) for x in c1]}
 
 
 def _sweep_program(n, B):

@@ -67,7 +67,13 @@ enum tcm_gate_kind {
   TCM_G_PHASE = 7,   /* diag(1, e^{i a})         gates.py:460-478 */
   TCM_G_EXP1 = 8,    /* cos(a) I - i sin(a) G, G = `mat` with G^2 = I,
                         nq in {1,2}              gates.py:825-858 */
-  TCM_G_BARRIER = 9  /* layer boundary (checkpoint candidate), no qubits */
+  TCM_G_BARRIER = 9, /* layer boundary (checkpoint candidate), no qubits */
+  TCM_G_EXPG = 10    /* exp(-i a G) = (I - G^2) + cos(a) G^2 - i sin(a) G for
+                        a Hermitian G = `mat` with G^3 = G (eigenvalues in
+                        {-1, 0, 1}), nq in {1,2}: controlled rotations
+                        (G = |1><1| (x) P / 2 scaled, gates.py:291-326),
+                        controlled phase (G = |11><11|), iswap(theta)
+                        (G = (XX + YY) / 2, gates.py:675-705) */
 };
 
 typedef struct tcm_gate {
@@ -198,6 +204,28 @@ int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
                         void* out_state, int64_t batch, const double* alpha,
                         void* stream);
 
+/* Per-term expectations: out[b][t] = w_t <psi_b|P_t|psi_b> as complex fp64
+ * (re, im) pairs, terms in the order given to tcm_pauli_build, all terms
+ * and states in one launch.  Replaces the reference's vmap of
+ * any_measurements over Pauli structures (templates/measurements.py:18-83,
+ * one expectation contraction per structure) and
+ * parameterized_local_measurements (templates/measurements.py:86-153).
+ * nterms and batch are each at most 65535.  Workspace:
+ * tcm_pauli_terms_workspace_bytes. */
+int tcm_pauli_terms_workspace_bytes(const tcm_pauli* ham, int64_t batch,
+                                    int64_t* bytes);
+int tcm_expect_pauli_terms(const tcm_pauli* ham, int32_t dtype,
+                           const void* state, int64_t batch, double* out,
+                           void* workspace, int64_t ws_bytes, void* stream);
+
+/* out_state_b = sum_t weights[b][t] w_t P_t psi_b (weights: device fp64
+ * [batch][nterms]) -- the vector-Jacobian product of
+ * tcm_expect_pauli_terms (the backward of any_measurements through the
+ * reference's autograd, basecircuit.py:267-319). */
+int tcm_apply_pauli_terms(const tcm_pauli* ham, int32_t dtype,
+                          const void* state, void* out_state, int64_t batch,
+                          const double* weights, void* stream);
+
 /* Sparse form of the sum: *nnz = nterms * 2^nqubits entries, written by
  * tcm_pauli_coo as rows / cols (device int64 [nnz]) and values (device
  * complex [nnz] of dtype): term t's entry t * 2^n + y sits at (y, y ^ flip)
@@ -217,6 +245,15 @@ int tcm_csr_expect(int32_t nqubits, int32_t dtype, const int64_t* rowptr,
                    const int64_t* cols, const void* vals, const void* state,
                    double* out, void* workspace, int64_t ws_bytes,
                    void* stream);
+
+/* y_b = A x_b for a 2^n x 2^n CSR matrix A (as tcm_csr_expect) and `batch`
+ * vectors x (device [batch][2^n] complex of dtype); x and y must not
+ * alias.  The vector-Jacobian product of sparse_expectation
+ * (templates/measurements.py:177-191): dL/dpsi = conj(g) A psi +
+ * g A^dagger psi, the second product with the CSR form of A^dagger. */
+int tcm_csr_matvec(int32_t nqubits, int32_t dtype, const int64_t* rowptr,
+                   const int64_t* cols, const void* vals, const void* x,
+                   void* y, int64_t batch, void* stream);
 
 /* out[b] = <a_b|b_b> (complex fp64) -- expectation of general operators after
  * they were applied to a copy of the state (circuit.py:914-990). */

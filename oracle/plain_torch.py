@@ -38,22 +38,38 @@ def _apply(psi: torch.Tensor, g: torch.Tensor, qubits: Sequence[int], n: int) ->
     return out.permute(perm)
 
 
-def hea_tfim_energy(params: torch.Tensor, n: int, nlayers: int, h: float = 1.0,
-                    dtype=torch.complex64) -> torch.Tensor:
-    """E(params) for params [2L, n]: ry/rz layers + CNOT chain, open-chain TFIM."""
-    X = torch.tensor([[0, 1], [1, 0]], dtype=dtype)
+def _layer(psi: torch.Tensor, ry: torch.Tensor, rz: torch.Tensor, n: int) -> torch.Tensor:
+    """One HEA layer: ry on every qubit, rz on every qubit, CNOT chain."""
+    dtype = psi.dtype
     Y = torch.tensor([[0, -1j], [1j, 0]], dtype=dtype)
     Z = torch.tensor([[1, 0], [0, -1]], dtype=dtype)
     CX = torch.tensor([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], dtype=dtype)
+    for i in range(n):
+        psi = _apply(psi, _rot(Y, ry[i]), (i,), n)
+    for i in range(n):
+        psi = _apply(psi, _rot(Z, rz[i]), (i,), n)
+    for i in range(n - 1):
+        psi = _apply(psi, CX, (i, i + 1), n)
+    return psi
+
+
+def hea_tfim_energy(params: torch.Tensor, n: int, nlayers: int, h: float = 1.0,
+                    dtype=torch.complex64, remat: bool = False) -> torch.Tensor:
+    """E(params) for params [2L, n]: ry/rz layers + CNOT chain, open-chain
+    TFIM.  ``remat``: each layer under torch.utils.checkpoint (its
+    intermediate states are recomputed in the backward pass instead of
+    stored), the reference's per-layer memory-saving pattern
+    (examples/checkpoint_memsave.py:41-104)."""
+    X = torch.tensor([[0, 1], [1, 0]], dtype=dtype)
+    Z = torch.tensor([[1, 0], [0, -1]], dtype=dtype)
     psi = torch.zeros([2] * n, dtype=dtype)
     psi.view(-1)[0] = 1.0
     for j in range(nlayers):
-        for i in range(n):
-            psi = _apply(psi, _rot(Y, params[2 * j, i]), (i,), n)
-        for i in range(n):
-            psi = _apply(psi, _rot(Z, params[2 * j + 1, i]), (i,), n)
-        for i in range(n - 1):
-            psi = _apply(psi, CX, (i, i + 1), n)
+        if remat:
+            psi = torch.utils.checkpoint.checkpoint(_layer, psi, params[2 * j], params[2 * j + 1], n,
+                                                    use_reentrant=False)
+        else:
+            psi = _layer(psi, params[2 * j], params[2 * j + 1], n)
     conj = torch.conj(psi)
     e = torch.zeros((), dtype=dtype)
     for i in range(n - 1):
@@ -64,15 +80,30 @@ def hea_tfim_energy(params: torch.Tensor, n: int, nlayers: int, h: float = 1.0,
     return torch.real(e)
 
 
-def time_energy_grad(n: int, nlayers: int, seed: int = 0, threads: int = 0) -> Tuple[float, float, List[float]]:
-    """Seconds for one value_and_grad of hea_tfim_energy (after one warm-up
-    at the same size if it is cheap); returns (seconds, energy, grad[:4])."""
+def time_energy_grad(n: int, nlayers: int, seed: int = 0, threads: int = 0,
+                     remat: bool = False) -> Tuple[float, float, List[float]]:
+    """Seconds for one energy + gradient (reverse-mode autograd, as
+    value_and_grad) of hea_tfim_energy; returns (seconds, energy, grad[:4])."""
+    import torch.utils.checkpoint  # noqa: F401
+
     if threads > 0:
         torch.set_num_threads(threads)
     rng = np.random.default_rng(seed)
-    p = torch.tensor(rng.uniform(0, 2 * np.pi, size=(2 * nlayers, n)).astype(np.float32))
-    f = torch.func.grad_and_value(lambda q: hea_tfim_energy(q, n, nlayers))
+    p = torch.tensor(rng.uniform(0, 2 * np.pi, size=(2 * nlayers, n)).astype(np.float32), requires_grad=True)
     t0 = time.perf_counter()
-    g, v = f(p)
+    v = hea_tfim_energy(p, n, nlayers, remat=remat)
+    v.backward()
     dt = time.perf_counter() - t0
-    return dt, float(v), [float(x) for x in g.reshape(-1)[:4]]
+    return dt, float(v.detach()), [float(x) for x in p.grad.reshape(-1)[:4]]
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"

@@ -17,7 +17,7 @@ TCM_C64 = 0
 TCM_C128 = 1
 
 # tcm_gate_kind
-G_MAT, G_RX, G_RY, G_RZ, G_RXX, G_RYY, G_RZZ, G_PHASE, G_EXP1, G_BARRIER = range(10)
+G_MAT, G_RX, G_RY, G_RZ, G_RXX, G_RYY, G_RZZ, G_PHASE, G_EXP1, G_BARRIER, G_EXPG = range(11)
 
 E_ARG, E_QUBIT, E_DUPQUBIT, E_WORKSPACE, E_UNSUPPORTED, E_NODEVICE, E_NONFINITE = (
     -1, -2, -3, -4, -5, -6, -7)
@@ -95,6 +95,10 @@ SIGNATURES = {
     "tcm_pauli_workspace_bytes": (C.c_int, [_vp, _i32, _i64, C.POINTER(_i64)]),
     "tcm_expect_pauli_sum": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tcm_apply_pauli_sum": (C.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp]),
+    "tcm_pauli_terms_workspace_bytes": (C.c_int, [_vp, _i64, C.POINTER(_i64)]),
+    "tcm_expect_pauli_terms": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tcm_apply_pauli_terms": (C.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp]),
+    "tcm_csr_matvec": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "tcm_inner_workspace_bytes": (C.c_int, [_i32, _i64, C.POINTER(_i64)]),
     "tcm_inner": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tcm_probability": (C.c_int, [_i32, _i32, _vp, _i64, _vp, _vp]),

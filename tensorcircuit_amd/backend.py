@@ -3,7 +3,6 @@ statevector path uses, with the semantics of the reference's
 PyTorchBackend (backends/pytorch_backend.py:194-711)."""
 from __future__ import annotations
 
-from functools import partial
 from typing import Any, Callable, Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -18,68 +17,97 @@ class PyTorchBackend:
     name = "pytorch"
 
     # ---------------------------------------------------------- transforms --
+    # All four transforms are views of one builder, ``_diff``: it turns f into
+    # a function returning (value, aux, grads) with grads in the caller's
+    # argnums shape (one tensor for an int, a tuple for a sequence).  The
+    # public semantics are those of backends/pytorch_backend.py:497-709:
+    #   grad(f)(x)                 -> grads            (has_aux: (grads, (aux,)))
+    #   value_and_grad(f)(x)       -> (value, grads)   (has_aux: ((value, aux), grads))
+    #   vmap(f, vec)(x)            -> f mapped over axis 0 of the args in vec
+    #   vvag(f, argnums, vec)(x)   -> vmap of value_and_grad, with the grads of
+    #                                 arguments outside vec summed over the batch
+
+    @staticmethod
+    def _positions(argnums: Union[int, Sequence[int]]) -> Tuple[Tuple[int, ...], bool]:
+        """(argument positions, True when a single int was given)."""
+        if isinstance(argnums, int):
+            return (argnums,), True
+        return tuple(int(a) for a in argnums), False
+
+    def _diff(self, f: Callable[..., Any], argnums: Union[int, Sequence[int]],
+              has_aux: bool) -> Callable[..., Tuple[Any, Any, Any]]:
+        pos, single = self._positions(argnums)
+        # torch.func differentiates with respect to the given positions; it
+        # returns one gradient for an int argnums and a tuple otherwise
+        tf_arg = pos[0] if single else pos
+        inner = torch.func.grad_and_value(f, argnums=tf_arg, has_aux=has_aux)
+
+        def run(*args: Any, **kws: Any) -> Tuple[Any, Any, Any]:
+            grads, out = inner(*args, **kws)
+            if has_aux:
+                value, aux = out
+                return value, aux, grads
+            return out, None, grads
+
+        return run
+
     def grad(self, f: Callable[..., Any], argnums: Union[int, Sequence[int]] = 0,
              has_aux: bool = False) -> Callable[..., Any]:
-        """pytorch_backend.py:497-509."""
+        """Gradient of a scalar f (pytorch_backend.py:497-509); with has_aux
+        the auxiliary output comes back as a 1-tuple beside the grads."""
+        run = self._diff(f, argnums, has_aux)
 
-        def wrapper(*args: Any, **kws: Any) -> Any:
-            y, gr = self.value_and_grad(f, argnums, has_aux)(*args, **kws)
-            if has_aux:
-                return gr, y[1:]
-            return gr
+        def grad_f(*args: Any, **kws: Any) -> Any:
+            _, aux, grads = run(*args, **kws)
+            return (grads, (aux,)) if has_aux else grads
 
-        return wrapper
+        return grad_f
 
     def value_and_grad(self, f: Callable[..., Any], argnums: Union[int, Sequence[int]] = 0,
                        has_aux: bool = False) -> Callable[..., Tuple[Any, Any]]:
-        """pytorch_backend.py:533-544 (torch.func.grad_and_value)."""
+        """(value, grads) of a scalar f (pytorch_backend.py:533-544); with
+        has_aux the value slot holds (value, aux)."""
+        run = self._diff(f, argnums, has_aux)
 
-        def wrapper(*args: Any, **kws: Any) -> Any:
-            gavf = torch.func.grad_and_value(f, argnums=argnums, has_aux=has_aux)
-            g, v = gavf(*args, **kws)
-            return v, g
+        def vag_f(*args: Any, **kws: Any) -> Tuple[Any, Any]:
+            value, aux, grads = run(*args, **kws)
+            return ((value, aux) if has_aux else value), grads
 
-        return wrapper
+        return vag_f
 
     def vmap(self, f: Callable[..., Any], vectorized_argnums: Union[int, Sequence[int]] = 0) -> Any:
-        """pytorch_backend.py:605-617."""
-        if isinstance(vectorized_argnums, int):
-            vectorized_argnums = (vectorized_argnums,)
+        """Map f over the leading axis of the vectorised positional arguments
+        (pytorch_backend.py:605-617); the others are broadcast."""
+        vec = frozenset(self._positions(vectorized_argnums)[0])
 
-        def wrapper(*args: Any, **kws: Any) -> Tensor:
-            in_axes = tuple([0 if i in vectorized_argnums else None for i in range(len(args))])
-            return torch.vmap(f, in_axes, 0)(*args, **kws)
+        def mapped(*args: Any, **kws: Any) -> Any:
+            dims = tuple(0 if k in vec else None for k in range(len(args)))
+            return torch.vmap(f, in_dims=dims, out_dims=0)(*args, **kws)
 
-        return wrapper
+        return mapped
 
     def vectorized_value_and_grad(self, f: Callable[..., Any], argnums: Union[int, Sequence[int]] = 0,
                                   vectorized_argnums: Union[int, Sequence[int]] = 0,
                                   has_aux: bool = False) -> Callable[..., Tuple[Any, Any]]:
-        """vmap(value_and_grad); grads of non-vectorized args are summed over
-        the batch (pytorch_backend.py:673-709)."""
-        if isinstance(vectorized_argnums, int):
-            vectorized_argnums = (vectorized_argnums,)
+        """vmap of value_and_grad; the gradient of an argument that is not
+        vectorised is the sum of its per-sample gradients
+        (pytorch_backend.py:673-709)."""
+        pos, single = self._positions(argnums)
+        vec = self._positions(vectorized_argnums)[0]
+        batched = self.vmap(self.value_and_grad(f, argnums, has_aux), vectorized_argnums=vec)
+        # which of the returned gradients need the batch axis summed away
+        fold = tuple(p not in vec for p in pos)
 
-        def wrapper(*args: Any, **kws: Any) -> Tuple[Tensor, Union[Tensor, Tuple[Tensor, ...]]]:
-            jf = self.value_and_grad(f, argnums=argnums, has_aux=has_aux)
-            jf = self.vmap(jf, vectorized_argnums=vectorized_argnums)
-            vs, gs = jf(*args, **kws)
-            if isinstance(argnums, int):
-                argnums_list = [argnums]
-                gs = [gs]
-            else:
-                argnums_list = list(argnums)
-                gs = list(gs)
-            for i, (j, g) in enumerate(zip(argnums_list, gs)):
-                if j not in vectorized_argnums:
-                    gs[i] = self.tree_map(partial(torch.sum, dim=0), g)
-            if isinstance(argnums, int):
-                gs = gs[0]
-            else:
-                gs = tuple(gs)
-            return vs, gs
+        def sum_batch(g: Any) -> Any:
+            return self.tree_map(lambda t: torch.sum(t, dim=0), g)
 
-        return wrapper
+        def vvag_f(*args: Any, **kws: Any) -> Tuple[Any, Any]:
+            values, grads = batched(*args, **kws)
+            if single:
+                return values, (sum_batch(grads) if fold[0] else grads)
+            return values, tuple(sum_batch(g) if s else g for g, s in zip(grads, fold))
+
+        return vvag_f
 
     vvag = vectorized_value_and_grad
 

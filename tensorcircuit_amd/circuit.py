@@ -35,14 +35,24 @@ class _Op:
 
 
 def _float_of(x: Any, what: str) -> float:
+    """A host float of a gate parameter the engine treats as a constant; a
+    tensor that requires a gradient (or is traced by torch.func) raises
+    instead of being silently detached."""
     if isinstance(x, torch.Tensor):
+        if x.requires_grad:
+            raise NotImplementedError(
+                "%s: this gate is evaluated on the host as a constant, so its parameters cannot carry a "
+                "gradient" % what)
         try:
             return float(x.detach().cpu().real.item())
         except Exception as e:  # functorch-wrapped tensors cannot be read
             raise NotImplementedError(
-                "%s: this gate is evaluated on the host and its angle cannot be a traced tensor; "
-                "use rx/ry/rz/rxx/ryy/rzz/phase/exp1 for trainable angles" % what) from e
+                "%s: this gate is evaluated on the host and its parameters cannot be traced tensors" % what) from e
     return float(np.real(x))
+
+
+def _any_tensor(vars: Dict[str, Any]) -> bool:
+    return any(isinstance(v, torch.Tensor) for v in vars.values())
 
 
 class Circuit:
@@ -88,8 +98,10 @@ class Circuit:
                 raise ValueError("qubit index %d out of range for %d qubits" % (i, self._nqubits))
         return idx
 
-    def _record(self, op: _Op, ir: Dict[str, Any]) -> None:
-        self._ops.append(op)
+    def _record(self, op: Union[_Op, List[_Op]], ir: Dict[str, Any]) -> None:
+        """Append one QIR entry and the engine ops it lowers to (one op, or
+        the factor list of a decomposed gate)."""
+        self._ops.append(op if isinstance(op, list) else [op])
         self._qir.append(ir)
         self.state_tensor = None  # refresh the state cache (basecircuit.py:245)
 
@@ -123,12 +135,26 @@ class Circuit:
             u = G.matrix_of(vars["unitary"])
             theta = vars.get("theta", 0.0)
             half = bool(vars.get("half", False)) if gname == "exp1" else False
-            if G.is_involution(u):
+            if G.is_involution(u) and len(idx) <= 2:
+                # exp1(U, t) = exp(-i t U) = cos t I - i sin t U for U^2 = I
                 self._record(_Op(_lib.G_EXP1, idx, theta=theta, scale=0.5 if half else 1.0, mat=u), ir)
+            elif gname == "exp" and G.is_generator(u) and len(idx) <= 2:
+                self._record(_Op(_lib.G_EXPG, idx, theta=theta, scale=1.0, mat=u), ir)
             else:
                 th = _float_of(theta, gname)
                 m = G.exp1_matrix(u, th, half) if gname == "exp1" else G.exp_matrix(u, th)
                 self._record(_Op(_lib.G_MAT, idx, mat=m), ir)
+            return
+        if gname in G.GENERATOR and _any_tensor(vars):
+            g, scale = G.GENERATOR[gname]
+            theta = vars.get("theta", G.PARAM_DEFAULT.get(gname, 0.0))
+            self._record(_Op(_lib.G_EXPG, idx, theta=theta, scale=scale, mat=g), ir)
+            return
+        if gname in G.DECOMPOSE and _any_tensor(vars):
+            ops = []
+            for kind, pos, pname, scale, g in G.DECOMPOSE[gname]:
+                ops.append(_Op(kind, tuple(idx[i] for i in pos), theta=vars.get(pname, 0.0), scale=scale, mat=g))
+            self._record(ops, ir)
             return
         if gname in ("any", "unitary"):
             m = G.matrix_of(vars["unitary"])
@@ -200,7 +226,11 @@ class Circuit:
 
         def apply(self, *index, **vars):
             localname = vars.pop("name", name)
-            vars.pop("split", None)
+            split = vars.pop("split", None)
+            if isinstance(gatef, G._GateFactory):
+                # one of the library's own gates: lowered (and differentiated) as such
+                self._apply_named(gatef.n, tuple(index), vars, localname, split)
+                return
             mat = gatef(**{k: (_float_of(v, name) if k != "unitary" else v) for k, v in vars.items()})
             idx = self._norm_index(index)
             ir = {"gatef": gatef, "index": idx, "target": list(idx), "name": localname, "split": None,
@@ -217,7 +247,7 @@ class Circuit:
     def _lower(self) -> Tuple[List[GateSpec], List[Any]]:
         specs: List[GateSpec] = []
         thetas: List[Any] = []
-        for op in self._ops:
+        for op in (o for entry in self._ops for o in entry):
             if op.kind == _lib.G_BARRIER:
                 specs.append(engine.barrier())
             elif op.kind == _lib.G_MAT:
@@ -558,19 +588,93 @@ class Circuit:
                     names.update(alias)
         return len([d for d in self._qir if str(d.get("name", "")).lower() in names])
 
-    def inverse(self) -> "Circuit":
-        """U^dagger as a new circuit (abstractcircuit.py:428-507)."""
-        c = Circuit(self._nqubits)
-        for op, ir in zip(reversed(self._ops), reversed(self._qir)):
-            if op.kind == _lib.G_BARRIER:
+    def inverse(self, circuit_params: Optional[Dict[str, Any]] = None) -> "Circuit":
+        """U^dagger as a new circuit of the same class, with the reference's
+        QIR rewriting (abstractcircuit.py:428-507): gates run in reverse
+        order; sd/td become s/t; r, cr and the other rotations take
+        theta -> -theta (iswap's default theta is 1); u and cu take
+        (phi, lbd) -> (pi - lbd, pi - phi); any / exp / exp1, wroot and
+        user gates become their adjoint (QIR name "<gate>d").  Unlike the
+        reference, which re-applies s and t unchanged, s and t become sd and
+        td, so that inverse() always inverts."""
+        params = dict(circuit_params or {})
+        params.setdefault("nqubits", self._nqubits)
+        c = type(self)(**params)
+        for ops, d in zip(reversed(self._ops), reversed(self._qir)):
+            gatef = d.get("gatef")
+            index = tuple(d.get("index", ()))
+            name, split = d.get("name"), d.get("split")
+            if gatef is None and name == "barrier":
                 c.barrier_instruction()
-            elif op.kind == _lib.G_MAT:
-                c._record(_Op(_lib.G_MAT, op.qubits, mat=np.asarray(op.mat).conj().T), dict(ir))
+                continue
+            gname = gatef.n if isinstance(gatef, G._GateFactory) else None
+            if gname in G.sgates and gname not in ("wroot",):
+                inv = {"sd": "s", "td": "t", "s": "sd", "t": "td"}.get(gname, gname)
+                c._apply_named(inv, index, {}, name, split)
+            elif gname in ("r", "cr") or (gname in G.vgates and gname not in ("u", "cu", "any", "exp", "exp1")):
+                vs = dict(d.get("parameters", {}))
+                vs["theta"] = -vs.get("theta", G.PARAM_DEFAULT.get(gname, 0.0))
+                c._apply_named(gname, index, vs, name, split)
+            elif gname in ("u", "cu"):
+                vs = dict(d.get("parameters", {}))
+                phi, lbd = vs.get("phi", 0.0), vs.get("lbd", 0.0)
+                vs["lbd"] = -phi + np.pi
+                vs["phi"] = -lbd + np.pi
+                c._apply_named(gname, index, vs, name, split)
             else:
-                th = op.theta
-                c._record(_Op(op.kind, op.qubits, theta=-th if not isinstance(th, (int, float)) else -float(th),
-                              scale=op.scale, mat=op.mat), dict(ir))
+                # adjoint of the gate as recorded: the inverse of each
+                # lowered op in reverse order
+                inv_ops = []
+                for op in reversed(ops):
+                    if op.kind == _lib.G_MAT:
+                        inv_ops.append(_Op(_lib.G_MAT, op.qubits, mat=np.asarray(op.mat).conj().T))
+                    else:
+                        inv_ops.append(_Op(op.kind, op.qubits, theta=-op.theta, scale=op.scale, mat=op.mat))
+                adj = G.adjoint_factory(gatef) if gatef is not None else None
+                ir = {"gatef": adj, "index": index, "target": list(index), "name": name, "split": split,
+                      "mpo": d.get("mpo", False)}
+                if "parameters" in d:
+                    ir["parameters"] = dict(d["parameters"])
+                if gatef is None and d.get("gate") is not None:
+                    ir["gate"] = G.Gate(G.matrix_of(d["gate"]).conj().T, name or "any")
+                c._record(inv_ops, ir)
         return c
+
+    def _entries(self, indices: Optional[Sequence[int]] = None):
+        """(ops, QIR entry) pairs of this circuit, qubits mapped through
+        ``indices`` (qubit i -> indices[i])."""
+        for ops, d in zip(self._ops, self._qir):
+            d = dict(d)
+            if indices is None:
+                yield list(ops), d
+                continue
+            m = [int(i) for i in indices]
+            if "index" in d:
+                d["index"] = tuple(m[i] for i in d["index"])
+                d["target"] = list(d["index"])
+            yield [_Op(o.kind, tuple(m[q] for q in o.qubits), theta=o.theta, scale=o.scale, mat=o.mat)
+                   for o in ops], d
+
+    def append(self, c: "Circuit", indices: Optional[List[int]] = None) -> "Circuit":
+        """Append the gates of ``c`` after this circuit's, qubit i of ``c``
+        on ``indices[i]`` (abstractcircuit.py:1148-1190); trainable gates
+        stay trainable."""
+        for ops, d in list(c._entries(indices)):
+            self._record(ops, d)
+        return self
+
+    def prepend(self, c: "Circuit") -> "Circuit":
+        """Put the gates of ``c`` before this circuit's (abstractcircuit.py
+        prepend)."""
+        mine = list(self._entries())
+        self._ops, self._qir = [], []
+        for ops, d in list(c._entries()) + mine:
+            self._record(ops, d)
+        return self
+
+    def copy(self) -> "Circuit":
+        c = type(self)(**{k: v for k, v in self.circuit_param.items() if k != "mps_inputs"})
+        return c.append(self)
 
     def matrix(self) -> Tensor:
         """The circuit unitary (circuit.py matrix): columns U|j> for every j."""

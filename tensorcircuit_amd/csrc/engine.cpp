@@ -21,6 +21,7 @@ struct tcm_program {
 
 struct PauliDev {
   void* terms = nullptr;
+  void* flat = nullptr;    // DPTermF per term in the caller's order
   void* groups = nullptr;  // DPGroup per flip mask (sparse form only)
   void* tpasses[2] = {nullptr, nullptr};
   void* tgroups[2] = {nullptr, nullptr};
@@ -54,6 +55,7 @@ struct tcm_pauli {
   int n = 0;
   std::vector<DPGroup> groups;     // by flip mask, diagonal first
   std::vector<DPTerm> terms;
+  std::vector<DPTermF> flat;       // every term in the caller's order
   int nslots = 0;                  // slots of the per-group kernels
   PauliTileSched ts[2];
   PauliJit jit[2];
@@ -274,6 +276,9 @@ int ensure_pauli(tcm_pauli* H, int dt, PauliDev** out) {
   }
   if (!pd.groups) {
     if (int rc = upload(H->groups, &pd.groups)) return rc;
+  }
+  if (!pd.flat) {
+    if (int rc = upload(H->flat, &pd.flat)) return rc;
   }
   const PauliTileSched& S = H->ts[dt];
   if (S.ok && !pd.tpasses[dt]) {
@@ -824,6 +829,7 @@ int tcm_pauli_build(const int8_t* codes, const double* coeffs, int32_t nterms, i
   if (nqubits < 1 || nqubits > 32) return set_error(TCM_E_ARG, "nqubits=%d out of range", nqubits);
   if (nterms < 1 || !codes || !coeffs) return set_error(TCM_E_ARG, "need at least one term");
   std::map<uint64_t, std::vector<DPTerm>> by_flip;
+  std::vector<DPTermF> flat;
   for (int t = 0; t < nterms; ++t) {
     if (!std::isfinite(coeffs[t])) return set_error(TCM_E_NONFINITE, "term %d: non-finite coefficient", t);
     uint64_t flip = 0, yz = 0;
@@ -841,9 +847,16 @@ int tcm_pauli_build(const int8_t* codes, const double* coeffs, int32_t nterms, i
     tm.ny = ny;
     tm.w = coeffs[t];
     by_flip[flip].push_back(tm);
+    DPTermF tf{};
+    tf.flip = flip;
+    tf.yz = yz;
+    tf.ny = ny;
+    tf.w = coeffs[t];
+    flat.push_back(tf);
   }
   tcm_pauli* H = new tcm_pauli();
   H->n = nqubits;
+  H->flat = std::move(flat);
   int slot = 0;
   for (auto& kv : by_flip) {  // std::map: flip 0 (diagonal) first
     DPGroup g{};
@@ -871,6 +884,7 @@ int tcm_pauli_free(tcm_pauli* ham) {
     PauliDev& pd = ham->dev[d];
     (void)hipSetDevice((int)d);
     if (pd.terms) (void)hipFree(pd.terms);
+    if (pd.flat) (void)hipFree(pd.flat);
     if (pd.groups) (void)hipFree(pd.groups);
     for (int dt = 0; dt < 2; ++dt) {
       void* ps[] = {pd.tpasses[dt], pd.tgroups[dt], pd.tterms[dt], pd.truns[dt]};
@@ -1096,6 +1110,79 @@ int tcm_apply_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state, 
                             : pauli_energy_apply<double>(H, dtype, state, out_state, alpha, batch, nullptr, false, true, st);
   if (rc) return rc;
   HIP_TRY(hipGetLastError(), "apply launch");
+  return 0;
+}
+
+// ---- per-term Pauli expectations / weighted applications ------------------
+namespace {
+int pauli_terms_blocks(int n) { return pauli_blocks(n, false); }
+int64_t pauli_terms_ws(const tcm_pauli* H, int64_t batch) {
+  return align_up(batch * (int64_t)H->flat.size() * pauli_terms_blocks(H->n) * 2 * (int64_t)sizeof(double));
+}
+}  // namespace
+
+int tcm_pauli_terms_workspace_bytes(const tcm_pauli* ham, int64_t batch, int64_t* bytes) {
+  if (!ham || !bytes || batch < 0) return set_error(TCM_E_ARG, "bad arguments to tcm_pauli_terms_workspace_bytes");
+  *bytes = pauli_terms_ws(ham, batch);
+  return 0;
+}
+
+int tcm_expect_pauli_terms(const tcm_pauli* ham, int32_t dtype, const void* state, int64_t batch, double* out,
+                           void* workspace, int64_t ws_bytes, void* stream) {
+  if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_expect_pauli_terms");
+  if (batch == 0) return 0;
+  const int64_t nt = (int64_t)ham->flat.size();
+  if (nt > 65535 || batch > 65535)
+    return set_error(TCM_E_ARG, "tcm_expect_pauli_terms: %lld terms x %lld states exceeds 65535 per axis",
+                     (long long)nt, (long long)batch);
+  if (!workspace || pauli_terms_ws(ham, batch) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  PauliDev* pd = nullptr;
+  if (int rc = ensure_pauli(H, dtype, &pd)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int nblk = pauli_terms_blocks(H->n);
+  if (dtype == TCM_C64)
+    launch_pauli_terms<float>((const DPTermF*)pd->flat, (int)nt, H->n, state, batch, nblk, (double*)workspace, out, st);
+  else
+    launch_pauli_terms<double>((const DPTermF*)pd->flat, (int)nt, H->n, state, batch, nblk, (double*)workspace, out,
+                               st);
+  HIP_TRY(hipGetLastError(), "pauli_terms launch");
+  return 0;
+}
+
+int tcm_apply_pauli_terms(const tcm_pauli* ham, int32_t dtype, const void* state, void* out_state, int64_t batch,
+                          const double* weights, void* stream) {
+  if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out_state || !weights)) ||
+      state == out_state)
+    return set_error(TCM_E_ARG, "bad arguments to tcm_apply_pauli_terms");
+  if (batch == 0) return 0;
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  PauliDev* pd = nullptr;
+  if (int rc = ensure_pauli(H, dtype, &pd)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int nt = (int)H->flat.size();
+  if (dtype == TCM_C64)
+    launch_pauli_terms_apply<float>((const DPTermF*)pd->flat, nt, H->n, state, out_state, weights, batch, st);
+  else
+    launch_pauli_terms_apply<double>((const DPTermF*)pd->flat, nt, H->n, state, out_state, weights, batch, st);
+  HIP_TRY(hipGetLastError(), "pauli_terms_apply launch");
+  return 0;
+}
+
+int tcm_csr_matvec(int32_t n, int32_t dtype, const int64_t* rowptr, const int64_t* cols, const void* vals,
+                   const void* x, void* y, int64_t batch, void* stream) {
+  if (n < 1 || n > 40 || !valid_dtype(dtype) || batch < 0 ||
+      (batch > 0 && (!rowptr || !cols || !vals || !x || !y)) || x == y)
+    return set_error(TCM_E_ARG, "bad arguments to tcm_csr_matvec");
+  if (batch == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nrows = (int64_t)1 << n;
+  if (dtype == TCM_C64)
+    launch_csr_matvec<float>(rowptr, cols, vals, x, y, nrows, batch, st);
+  else
+    launch_csr_matvec<double>(rowptr, cols, vals, x, y, nrows, batch, st);
+  HIP_TRY(hipGetLastError(), "csr_matvec launch");
   return 0;
 }
 

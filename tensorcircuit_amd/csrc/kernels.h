@@ -46,6 +46,16 @@ struct DPPass {
   int32_t slot0;       // first energy-partial slot
   int32_t blk_bits[32];
 };
+// one Pauli term in the caller's order (per-term expectations / weighted
+// applications): P = w (-i)^ny X^flip with the sign (-1)^popcount(row & yz)
+struct DPTermF {
+  uint64_t flip;   // X/Y positions (index bits)
+  uint64_t yz;     // Y/Z positions (index bits)
+  int32_t ny;      // number of Y factors
+  int32_t pad;
+  double w;        // weight
+};
+
 struct DPGroupT {
   uint32_t flip_t;     // tile-local X/Y mask
   int32_t t0, nt, pad;
@@ -159,6 +169,17 @@ void launch_probability(const void* psi, void* out, int64_t total, hipStream_t s
 template <typename T>
 void launch_pauli_coo(const DPGroup* groups, int ngroups, const DPTerm* terms, int n, int64_t nterms, int64_t* rows,
                       int64_t* cols, void* vals, hipStream_t st);
+// per-term Pauli expectations and weighted applications, CSR mat-vec
+// (kernels_aux.hip)
+template <typename T>
+void launch_pauli_terms(const DPTermF* terms_dev, int nterms, int n, const void* psi, int64_t batch, int nblk,
+                        double* part, double* out, hipStream_t st);
+template <typename T>
+void launch_pauli_terms_apply(const DPTermF* terms_dev, int nterms, int n, const void* psi, void* out,
+                              const double* weights, int64_t batch, hipStream_t st);
+template <typename T>
+void launch_csr_matvec(const int64_t* rowptr, const int64_t* cols, const void* vals, const void* x, void* y,
+                       int64_t nrows, int64_t batch, hipStream_t st);
 template <typename T>
 void launch_csr_expect(const int64_t* rowptr, const int64_t* cols, const void* vals, const void* psi, int64_t nrows,
                        int nblk, double* part, double* out, hipStream_t st);

@@ -629,6 +629,28 @@ __global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, const DFactor*
             }
         }
       } break;
+      case TCM_G_EXPG: {
+        // (I - Q) + cos(a) Q - i sin(a) G; G then Q = G^2 in the constant table
+        double c, s;
+        sincos(a, &s, &c);
+        const int d = (r.layout == OK_U1 || r.layout == OK_D1) ? 2 : 4;
+        const T* g = ct + r.goff;
+        const T* q = g + 2 * d * d;
+        if (r.layout == OK_D1 || r.layout == OK_D2) {
+          for (int k = 0; k < d; ++k) {
+            const int e = 2 * (k * d + k);
+            o[2 * k] = (T)(1.0 + (c - 1.0) * (double)q[e] + s * (double)g[e + 1]);
+            o[2 * k + 1] = (T)((c - 1.0) * (double)q[e + 1] - s * (double)g[e]);
+          }
+        } else {
+          for (int rr = 0; rr < d; ++rr)
+            for (int cc = 0; cc < d; ++cc) {
+              const int e = 2 * (rr * d + cc);
+              o[e] = (T)((rr == cc ? 1.0 : 0.0) + (c - 1.0) * (double)q[e] + s * (double)g[e + 1]);
+              o[e + 1] = (T)((c - 1.0) * (double)q[e + 1] - s * (double)g[e]);
+            }
+        }
+      } break;
       default: break;
     }
   }

@@ -325,47 +325,61 @@ struct Gen {
   std::string pv(int j) const { return var("p", j); }
   std::string lv(int j) const { return var("l", j); }
 
+  // ga += (or -=) Im(conj(a) b) = a.x b.y - a.y b.x, and Re(conj(a) b)
+  void imcj_acc(const std::string& a, const std::string& b, bool plus) {
+    line(fmt("ga %s= %s.x * %s.y; ga %s= %s.y * %s.x;", plus ? "+" : "-", a.c_str(), b.c_str(), plus ? "-" : "+",
+             a.c_str(), b.c_str()));
+  }
+  void recj_acc(const std::string& a, const std::string& b, bool plus) {
+    const char* sg = plus ? "+" : "-";
+    line(fmt("ga %s= %s.x * %s.x; ga %s= %s.y * %s.y;", sg, a.c_str(), b.c_str(), sg, a.c_str(), b.c_str()));
+  }
+
   // gradient Im<l|G|p> of a generator on the registers; returns expression code into `ga`
   bool grad(int gen, int s0, int s1, int goff) {
     line("T ga = 0;");
     switch (gen) {
+      // every term is one `ga += a * b` / `ga -= a * b` statement, so the
+      // compiler emits one fused multiply-add per product (a chain into ga)
+      // instead of forming each complex product separately
       case GN_X: {
         const int m = 1 << s0;
         for (int j = 0; j < nreg; ++j)
-          if (!(j & m))
-            line(fmt("ga += imcj(%s, %s) + imcj(%s, %s);", lv(j).c_str(), pv(j | m).c_str(), lv(j | m).c_str(),
-                     pv(j).c_str()));
+          if (!(j & m)) {
+            imcj_acc(lv(j), pv(j | m), true);
+            imcj_acc(lv(j | m), pv(j), true);
+          }
       } break;
       case GN_Y: {
         const int m = 1 << s0;
         for (int j = 0; j < nreg; ++j)
-          if (!(j & m))
-            line(fmt("ga += recj(%s, %s) - recj(%s, %s);", lv(j | m).c_str(), pv(j).c_str(), lv(j).c_str(),
-                     pv(j | m).c_str()));
+          if (!(j & m)) {
+            recj_acc(lv(j | m), pv(j), true);
+            recj_acc(lv(j), pv(j | m), false);
+          }
       } break;
       case GN_Z:
-        for (int j = 0; j < nreg; ++j)
-          line(fmt("ga %s= imcj(%s, %s);", (j >> s0 & 1) ? "-" : "+", lv(j).c_str(), pv(j).c_str()));
+        for (int j = 0; j < nreg; ++j) imcj_acc(lv(j), pv(j), !(j >> s0 & 1));
         break;
       case GN_P1:
         for (int j = 0; j < nreg; ++j)
-          if (j >> s0 & 1) line(fmt("ga += imcj(%s, %s);", lv(j).c_str(), pv(j).c_str()));
+          if (j >> s0 & 1) imcj_acc(lv(j), pv(j), true);
         break;
       case GN_XX: {
         const int m = (1 << s0) | (1 << s1);
-        for (int j = 0; j < nreg; ++j) line(fmt("ga += imcj(%s, %s);", lv(j).c_str(), pv(j ^ m).c_str()));
+        for (int j = 0; j < nreg; ++j) imcj_acc(lv(j), pv(j ^ m), true);
       } break;
       case GN_YY: {
         const int m = (1 << s0) | (1 << s1);
         for (int j = 0; j < nreg; ++j) {
           const int par = ((j >> s0) ^ (j >> s1)) & 1;
-          line(fmt("ga %s= imcj(%s, %s);", par ? "+" : "-", lv(j).c_str(), pv(j ^ m).c_str()));
+          imcj_acc(lv(j), pv(j ^ m), par != 0);
         }
       } break;
       case GN_ZZ:
         for (int j = 0; j < nreg; ++j) {
           const int par = ((j >> s0) ^ (j >> s1)) & 1;
-          line(fmt("ga %s= imcj(%s, %s);", par ? "-" : "+", lv(j).c_str(), pv(j).c_str()));
+          imcj_acc(lv(j), pv(j), par == 0);
         }
         break;
       case GN_M1: {

@@ -83,6 +83,37 @@ struct RefInit {
 } ref_init;
 
 // G^2 == I check for exp1 generators
+// a * b for interleaved complex d x d matrices
+std::vector<double> matmul_c(const std::vector<double>& a, const std::vector<double>& b, int d) {
+  std::vector<double> out(2 * d * d, 0.0);
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c < d; ++c) {
+      double sr = 0, si = 0;
+      for (int k = 0; k < d; ++k) {
+        const double ar = a[2 * (r * d + k)], ai = a[2 * (r * d + k) + 1];
+        const double br = b[2 * (k * d + c)], bi = b[2 * (k * d + c) + 1];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+      out[2 * (r * d + c)] = sr;
+      out[2 * (r * d + c) + 1] = si;
+    }
+  return out;
+}
+bool close_all(const std::vector<double>& a, const std::vector<double>& b, double tol) {
+  for (size_t i = 0; i < a.size(); ++i)
+    if (std::fabs(a[i] - b[i]) > tol) return false;
+  return true;
+}
+bool is_hermitian(const std::vector<double>& m, int d) {
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c < d; ++c)
+      if (std::fabs(m[2 * (r * d + c)] - m[2 * (c * d + r)]) > 1e-9 ||
+          std::fabs(m[2 * (r * d + c) + 1] + m[2 * (c * d + r) + 1]) > 1e-9)
+        return false;
+  return true;
+}
+
 bool is_involution(const std::vector<double>& g, int d) {
   for (int r = 0; r < d; ++r)
     for (int c = 0; c < d; ++c) {
@@ -579,6 +610,26 @@ static int lower(Program& P, const tcm_gate* gates, int32_t ngates) {
         if (g.param >= 0 && !is_involution(L.mat, d))
           return set_error(TCM_E_UNSUPPORTED,
                            "gate %d: trainable exp1 needs G^2 = I for an exact gradient", gi);
+        c = 1.0;
+        break;
+      }
+      case TCM_G_EXPG: {
+        // exp(-i a G), G Hermitian with G^3 = G; the constant table holds G
+        // then Q = G^2 (the matrix is (I - Q) + cos(a) Q - i sin(a) G and
+        // dU/da = -i G U, so G is the gradient generator)
+        if (g.nq > 2) return set_error(TCM_E_UNSUPPORTED, "gate %d: generator gate on %d qubits", gi, g.nq);
+        if (!g.mat) return set_error(TCM_E_ARG, "gate %d: EXPG without generator", gi);
+        if (!finite_all(g.mat, msz)) return set_error(TCM_E_NONFINITE, "gate %d: non-finite generator", gi);
+        L.mat.assign(g.mat, g.mat + msz);
+        std::vector<double> q = matmul_c(L.mat, L.mat, d);
+        if (!is_hermitian(L.mat, d) || !close_all(matmul_c(q, L.mat, d), L.mat, 1e-9))
+          return set_error(TCM_E_ARG, "gate %d: EXPG generator must be Hermitian with G^3 = G", gi);
+        const bool dg = is_diag(L.mat, d);
+        L.okind = g.nq == 1 ? (dg ? OK_D1 : OK_U1) : (dg ? OK_D2 : OK_U2);
+        L.gen = g.nq == 1 ? GN_M1 : GN_M2;
+        L.goff = (int)P.ctab.size();
+        P.ctab.insert(P.ctab.end(), L.mat.begin(), L.mat.end());
+        P.ctab.insert(P.ctab.end(), q.begin(), q.end());
         c = 1.0;
         break;
       }

@@ -52,6 +52,17 @@ def _workspace(nbytes: int, dev: torch.device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
 
 
+# objects whose device tables a HIP graph under capture reads (CapturedStep
+# holds them for the graph's lifetime); one stack per thread
+_SCOPE = threading.local()
+
+
+def _hold(*objs: Any) -> None:
+    stack = getattr(_SCOPE, "stack", None)
+    if stack:
+        stack[-1].extend(o for o in objs if o is not None)
+
+
 def _i64() -> C.c_int64:
     return C.c_int64(0)
 
@@ -139,6 +150,7 @@ class Program:
                 inputs: Optional[torch.Tensor] = None, want_ckpts: bool = False,
                 param_stride: Optional[int] = None,
                 out: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+        _hold(self)
         dev = device()
         N = 1 << self.n
         code = _CODE[dtype]
@@ -168,6 +180,7 @@ class Program:
     def vjp(self, params: Optional[torch.Tensor], ckpts: Optional[torch.Tensor], state: torch.Tensor,
             grad_state: torch.Tensor, batch: int, dtype: str, want_input: bool,
             param_stride: Optional[int] = None) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
+        _hold(self)
         dev = device()
         code = _CODE[dtype]
         p = None if self.nparams == 0 else params.to(device=dev, dtype=_RDT[dtype]).contiguous()
@@ -189,6 +202,7 @@ class Program:
     def energy_grad(self, params: torch.Tensor, ham: "PauliSum", batch: int, dtype: str = "complex64",
                     inputs: Optional[torch.Tensor] = None,
                     param_stride: Optional[int] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+        _hold(self, ham)
         dev = device()
         code = _CODE[dtype]
         p = params.to(device=dev, dtype=_RDT[dtype]).contiguous() if self.nparams else None
@@ -215,6 +229,7 @@ def forward_multi(items: Sequence[Tuple[Program, Optional[torch.Tensor], torch.T
                   dtype: str = "complex64", init_zero: bool = True) -> None:
     """Run many programs (e.g. circuits of different qubit counts) on their
     states [batch, 2^n] in pass-synchronous launches (tcm_forward_multi)."""
+    _hold(*(it[0] for it in items))
     dev = device()
     code = _CODE[dtype]
     descs = (_lib.TcmMultiDesc * max(len(items), 1))()
@@ -380,6 +395,7 @@ class PauliSum:
 
     def expect(self, state: torch.Tensor) -> torch.Tensor:
         """complex128 [batch] of <psi|H|psi> (fp64 accumulation)."""
+        _hold(self)
         dev = device()
         code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
         self._prepare(code)
@@ -395,6 +411,7 @@ class PauliSum:
 
     def apply(self, state: torch.Tensor, alpha: Optional[torch.Tensor] = None) -> torch.Tensor:
         """alpha_b * H psi_b as a new state tensor."""
+        _hold(self)
         code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
         self._prepare(code)
         s = state.reshape(-1, 1 << self.n).contiguous()
@@ -402,6 +419,36 @@ class PauliSum:
         a = None if alpha is None else alpha.to(torch.float64).contiguous()
         check(lib.tcm_apply_pauli_sum(self.handle, code, s.data_ptr(), out.data_ptr(), s.shape[0], _ptr(a),
                                       _stream()), "tcm_apply_pauli_sum")
+        return out
+
+
+    def expect_terms(self, state: torch.Tensor) -> torch.Tensor:
+        """complex128 [batch, nterms]: w_t <psi_b|P_t|psi_b> for every term, in
+        the order given, from one launch (tcm_expect_pauli_terms)."""
+        _hold(self)
+        dev = device()
+        code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
+        s = state.reshape(-1, 1 << self.n).contiguous()
+        b = s.shape[0]
+        nt = self.codes.shape[0]
+        out = torch.empty(b, nt, 2, dtype=torch.float64, device=dev)
+        nb = _i64()
+        check(lib.tcm_pauli_terms_workspace_bytes(self.handle, b, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, dev)
+        check(lib.tcm_expect_pauli_terms(self.handle, code, s.data_ptr(), b, out.data_ptr(), ws.data_ptr(),
+                                         ws.numel(), _stream()), "tcm_expect_pauli_terms")
+        return torch.view_as_complex(out)
+
+    def apply_terms(self, state: torch.Tensor, weights: torch.Tensor) -> torch.Tensor:
+        """sum_t weights[b, t] w_t P_t psi_b as a new state tensor
+        (tcm_apply_pauli_terms); weights real [batch, nterms]."""
+        _hold(self)
+        code = _CODE["complex64" if state.dtype == torch.complex64 else "complex128"]
+        s = state.reshape(-1, 1 << self.n).contiguous()
+        w = weights.to(device=s.device, dtype=torch.float64).reshape(s.shape[0], self.codes.shape[0]).contiguous()
+        out = torch.empty_like(s)
+        check(lib.tcm_apply_pauli_terms(self.handle, code, s.data_ptr(), out.data_ptr(), s.shape[0], w.data_ptr(),
+                                        _stream()), "tcm_apply_pauli_terms")
         return out
 
 
@@ -496,22 +543,77 @@ def pauli_coo(ham: "PauliSum", dtype: str = "complex64") -> torch.Tensor:
     return torch.sparse_coo_tensor(torch.stack([rows, cols]), vals, (N, N)).coalesce()
 
 
-def csr_expectation(h: torch.Tensor, state: torch.Tensor, n: int) -> torch.Tensor:
-    """complex <psi|H|psi> for a sparse (COO or CSR) H by tcm_csr_expect."""
-    s, code = _states(state, n)
-    hc = h.to(device=s.device, dtype=s.dtype)
-    if hc.layout != torch.sparse_csr:
-        hc = hc.coalesce().to_sparse_csr() if hc.layout == torch.sparse_coo else hc.to_sparse_csr()
-    rowptr = hc.crow_indices().to(torch.int64).contiguous()
-    cols = hc.col_indices().to(torch.int64).contiguous()
-    vals = hc.values().contiguous()
-    out = torch.empty(2, dtype=torch.float64, device=s.device)
-    nb = _i64()
-    check(lib.tcm_csr_expect_workspace_bytes(n, C.byref(nb)), "workspace")
-    ws = _workspace(nb.value, s.device)
-    check(lib.tcm_csr_expect(n, code, rowptr.data_ptr(), cols.data_ptr(), vals.data_ptr(), s.data_ptr(),
-                             out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()), "tcm_csr_expect")
-    return torch.complex(out[0], out[1])
+class CsrMatrix:
+    """A 2^n x 2^n sparse matrix on the device in CSR form (int64 row
+    pointers and columns, complex values of the state dtype), with the CSR
+    form of its conjugate transpose for the backward pass."""
+
+    def __init__(self, h: torch.Tensor, n: int, dtype: torch.dtype):
+        # format conversions of a constant matrix: run with torch.func's
+        # transforms off, so that a matrix captured from outside a grad /
+        # vmap stays a plain tensor
+        import warnings
+
+        with torch._C._DisableFuncTorch(), warnings.catch_warnings():
+            warnings.simplefilter("ignore", UserWarning)  # CSR support is "beta" in torch
+            self._build(h, n, dtype)
+
+    def _build(self, h: torch.Tensor, n: int, dtype: torch.dtype) -> None:
+        dev = device()
+        hc = h.detach().to(device=dev, dtype=dtype)
+        if hc.layout == torch.sparse_coo:
+            hc = hc.coalesce()
+        elif hc.layout != torch.sparse_csr:
+            hc = hc.to_sparse()
+        N = 1 << n
+        if tuple(hc.shape) != (N, N):
+            raise ValueError("sparse matrix of shape %s for %d qubits" % (tuple(hc.shape), n))
+        self.n = n
+        self.dtype = dtype
+        self.fwd = self._csr(hc)
+        # A^dagger: swap the coordinates of the COO form, conjugate the values
+        coo = hc.to_sparse_coo().coalesce() if hc.layout != torch.sparse_coo else hc
+        idx = coo.indices()
+        dag = torch.sparse_coo_tensor(torch.stack([idx[1], idx[0]]), torch.conj(coo.values()).resolve_conj(),
+                                      (N, N)).coalesce()
+        self.dag = self._csr(dag)
+
+    @staticmethod
+    def _csr(h: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        c = h.to_sparse_csr() if h.layout != torch.sparse_csr else h
+        return (c.crow_indices().to(torch.int64).contiguous(), c.col_indices().to(torch.int64).contiguous(),
+                c.values().contiguous())
+
+    def expect(self, st: torch.Tensor) -> torch.Tensor:
+        """complex128 <psi|A|psi> (tcm_csr_expect, fp64 accumulation)."""
+        _hold(self)
+        s, code = _states(st, self.n)
+        rowptr, cols, vals = self.fwd
+        out = torch.empty(2, dtype=torch.float64, device=s.device)
+        nb = _i64()
+        check(lib.tcm_csr_expect_workspace_bytes(self.n, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, s.device)
+        check(lib.tcm_csr_expect(self.n, code, rowptr.data_ptr(), cols.data_ptr(), vals.data_ptr(), s.data_ptr(),
+                                 out.data_ptr(), ws.data_ptr(), ws.numel(), _stream()), "tcm_csr_expect")
+        return torch.complex(out[0], out[1])
+
+    def matvec(self, x: torch.Tensor, dagger: bool = False) -> torch.Tensor:
+        """A x_b (or A^dagger x_b) for x [batch, 2^n] (tcm_csr_matvec)."""
+        _hold(self)
+        s, code = _states(x, self.n)
+        rowptr, cols, vals = self.dag if dagger else self.fwd
+        out = torch.empty_like(s)
+        check(lib.tcm_csr_matvec(self.n, code, rowptr.data_ptr(), cols.data_ptr(), vals.data_ptr(), s.data_ptr(),
+                                 out.data_ptr(), s.shape[0], _stream()), "tcm_csr_matvec")
+        return out
+
+
+def csr_expectation(h: Any, state: torch.Tensor, n: int) -> torch.Tensor:
+    """complex <psi|H|psi> for a sparse (COO or CSR) H by tcm_csr_expect,
+    differentiable in psi (CsrExpFn)."""
+    st = state.reshape(-1)
+    m = h if isinstance(h, CsrMatrix) else CsrMatrix(h, n, st.dtype)
+    return CsrExpFn.apply(st, m)
 
 
 def sample_prob_indices(p: torch.Tensor, n: int, uniforms: torch.Tensor) -> torch.Tensor:
@@ -774,6 +876,125 @@ class OpExpFn(torch.autograd.Function):
         return out, None, None, None
 
 
+class CsrExpFn(torch.autograd.Function):
+    """psi [N] -> <psi|A|psi> for a sparse A (complex, the state dtype);
+    dL/dpsi = conj(g) A psi + g A^dagger psi (the same convention as
+    OpExpFn), both products by the CSR mat-vec kernel."""
+
+    @staticmethod
+    def forward(st, m):
+        return m.expect(st).to(st.dtype).reshape(())
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, m = inputs
+        ctx.save_for_backward(st)
+        ctx.m = m
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        return CsrVJPFn.apply(st, g, ctx.m), None
+
+    @staticmethod
+    def vmap(info, in_dims, st, m):
+        B = info.batch_size
+        s = _to_batch(st, in_dims[0], B, (1 << m.n,))
+        return torch.stack([CsrExpFn.apply(s[b], m) for b in range(B)]), 0
+
+
+class CsrVJPFn(torch.autograd.Function):
+    @staticmethod
+    def forward(st, g, m):
+        s = st.reshape(1, -1)
+        gg = g.to(st.dtype).reshape(())
+        return (torch.conj(gg) * m.matvec(s) + gg * m.matvec(s, dagger=True)).reshape(st.shape)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, *grads):
+        raise NotImplementedError("higher-order derivatives of sparse expectations are not supported")
+
+    @staticmethod
+    def vmap(info, in_dims, st, g, m):
+        B = info.batch_size
+        N = 1 << m.n
+        s = _to_batch(st, in_dims[0], B, (N,))
+        gg = _to_batch(g, in_dims[1], B, ()).to(s.dtype).reshape(B, 1)
+        return torch.conj(gg) * m.matvec(s) + gg * m.matvec(s, dagger=True), 0
+
+
+class PauliTermsFn(torch.autograd.Function):
+    """psi [N] -> [nterms] of w_t <psi|P_t|psi> (complex, the state dtype)
+    from one launch; backward sum_t 2 Re(g_t) w_t P_t psi (one launch)."""
+
+    @staticmethod
+    def forward(st, ham):
+        return ham.expect_terms(st.reshape(1, -1)).to(st.dtype).reshape(-1)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, ham = inputs
+        ctx.save_for_backward(st)
+        ctx.ham = ham
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        return PauliTermsVJPFn.apply(st, g, ctx.ham), None
+
+    @staticmethod
+    def vmap(info, in_dims, st, ham):
+        B = info.batch_size
+        s = _to_batch(st, in_dims[0], B, (1 << ham.n,))
+        return PauliTermsBatchFn.apply(s, ham), 0
+
+
+class PauliTermsBatchFn(torch.autograd.Function):
+    """psi [B, N] -> [B, nterms]; backward sum_t 2 Re(g_bt) w_t P_t psi_b."""
+
+    @staticmethod
+    def forward(st, ham):
+        return ham.expect_terms(st).to(st.dtype)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        st, ham = inputs
+        ctx.save_for_backward(st)
+        ctx.ham = ham
+
+    @staticmethod
+    def backward(ctx, g):
+        (st,) = ctx.saved_tensors
+        return ctx.ham.apply_terms(st, 2.0 * torch.real(g)), None
+
+
+class PauliTermsVJPFn(torch.autograd.Function):
+    @staticmethod
+    def forward(st, g, ham):
+        w = (2.0 * torch.real(g)).reshape(1, -1)
+        return ham.apply_terms(st.reshape(1, -1), w).reshape(st.shape)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        pass
+
+    @staticmethod
+    def backward(ctx, *grads):
+        raise NotImplementedError("higher-order derivatives of expectations are not supported")
+
+    @staticmethod
+    def vmap(info, in_dims, st, g, ham):
+        B = info.batch_size
+        N = 1 << ham.n
+        s = _to_batch(st, in_dims[0], B, (N,))
+        gg = _to_batch(g, in_dims[1], B, (ham.codes.shape[0],))
+        return ham.apply_terms(s, 2.0 * torch.real(gg)), 0
+
+
 class ProbFn(torch.autograd.Function):
     """psi [..., N] -> |psi|^2 (tcm_probability); backward 2 g psi (the torch
     convention for real losses of complex inputs)."""
@@ -814,20 +1035,36 @@ class CapturedStep:
 
     The callable runs ``warmup`` times eagerly first (on a side stream of
     default priority) so that device tables, compiled kernels and workspace
-    sizes exist before capture; the entry points record no memset nodes."""
+    sizes exist before capture; the entry points record no memset nodes.
+
+    Lifetime: the graph reads the device tables and compiled kernels of
+    every Program, PauliSum and CsrMatrix the callable used during capture.
+    The step holds references to all of them (and to the callable), so the
+    engine's caches may evict them without freeing memory the graph still
+    uses; they are released when the step is."""
 
     def __init__(self, fn, warmup: int = 2):
         device()
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            for _ in range(max(1, warmup)):
-                fn()
-        torch.cuda.current_stream().wait_stream(side)
-        torch.cuda.synchronize()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.out = fn()
+        self.fn = fn
+        stack = getattr(_SCOPE, "stack", None)
+        if stack is None:
+            stack = _SCOPE.stack = []
+        stack.append([])
+        try:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                for _ in range(max(1, warmup)):
+                    fn()
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self.out = fn()
+        finally:
+            used = stack.pop()
+        # one reference per distinct object
+        self.held = list({id(o): o for o in used}.values())
 
     def replay(self):
         self.graph.replay()

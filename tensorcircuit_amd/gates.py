@@ -160,6 +160,81 @@ HOST_PARAM = {
 }
 
 
+# --- trainable lowering of the other parametrised gates ----------------------
+# Every parametrised gate is a product of factors exp(-i scale * theta * G)
+# with G Hermitian and G^3 = G, so that dU/dtheta = -i scale G U and the
+# adjoint sweep differentiates it like a rotation.
+
+_p0_matrix = np.diag([1.0, 0.0])
+_p1_matrix = np.diag([0.0, 1.0])
+_p11_matrix = np.diag([0.0, 0.0, 0.0, 1.0])
+_hop_matrix = (_xx_matrix + _yy_matrix).real / 2  # |01><10| + |10><01|
+
+# single-parameter gates on their own qubits: name -> (G, scale)
+#   c<r>(theta) = |0><0| (x) I + |1><1| (x) exp(-i theta/2 P) = exp(-i theta/2 |1><1| (x) P)
+#   o<r>(theta): the same with |0><0| (gates.py:309-326)
+#   cphase(theta) = diag(1, 1, 1, e^{i theta}) = exp(+i theta |11><11|)
+#   iswap(theta) = exp(+i theta pi/2 (|01><10| + |10><01|))   (gates.py:675-705)
+GENERATOR = {
+    "crx": (np.kron(_p1_matrix, _x_matrix), 0.5),
+    "cry": (np.kron(_p1_matrix, _y_matrix), 0.5),
+    "crz": (np.kron(_p1_matrix, _z_matrix), 0.5),
+    "orx": (np.kron(_p0_matrix, _x_matrix), 0.5),
+    "ory": (np.kron(_p0_matrix, _y_matrix), 0.5),
+    "orz": (np.kron(_p0_matrix, _z_matrix), 0.5),
+    "cphase": (_p11_matrix, -1.0),
+    "iswap": (_hop_matrix, -np.pi / 2),
+}
+
+# multi-parameter gates as ordered factor lists (first factor applied first):
+# (tcm kind, positions within the gate's qubits, parameter name, scale, G)
+#   u(theta, phi, lbd) = phase(phi) ry(theta) phase(lbd)        (gates.py:509-540)
+#   cu = the same factors controlled: cphase(phi) cry(theta) cphase(lbd)
+#   r(theta, alpha, phi) = exp(-i theta n.sigma), n = (cos phi sin alpha,
+#     sin phi sin alpha, cos alpha) = V rz(2 theta) V^dagger with
+#     V = rz(phi) ry(alpha)                                     (gates.py:545-575)
+#   cr = (I (x) V) crz(2 theta) (I (x) V^dagger)                 (gates.py:720-745)
+DECOMPOSE = {
+    "u": [(_lib.G_PHASE, (0,), "lbd", 1.0, None), (_lib.G_RY, (0,), "theta", 1.0, None),
+          (_lib.G_PHASE, (0,), "phi", 1.0, None)],
+    "cu": [(_lib.G_EXPG, (0, 1), "lbd", -1.0, _p11_matrix),
+           (_lib.G_EXPG, (0, 1), "theta", 0.5, np.kron(_p1_matrix, _y_matrix)),
+           (_lib.G_EXPG, (0, 1), "phi", -1.0, _p11_matrix)],
+    "r": [(_lib.G_RZ, (0,), "phi", -1.0, None), (_lib.G_RY, (0,), "alpha", -1.0, None),
+          (_lib.G_RZ, (0,), "theta", 2.0, None), (_lib.G_RY, (0,), "alpha", 1.0, None),
+          (_lib.G_RZ, (0,), "phi", 1.0, None)],
+    "cr": [(_lib.G_RZ, (1,), "phi", -1.0, None), (_lib.G_RY, (1,), "alpha", -1.0, None),
+           (_lib.G_EXPG, (0, 1), "theta", 1.0, np.kron(_p1_matrix, _z_matrix)),
+           (_lib.G_RY, (1,), "alpha", 1.0, None), (_lib.G_RZ, (1,), "phi", 1.0, None)],
+}
+
+# parameter defaults of the reference's gate factories
+PARAM_DEFAULT = {"iswap": 1.0}
+
+
+def factor_matrix(kind: int, g: Optional[np.ndarray], a: float) -> np.ndarray:
+    """Host matrix of one lowered factor at angle a = scale * theta (the
+    engine's conventions, include/tcm.h)."""
+    if kind == _lib.G_RX:
+        return _rot(_x_matrix, a)
+    if kind == _lib.G_RY:
+        return _rot(_y_matrix, a)
+    if kind == _lib.G_RZ:
+        return _rot(_z_matrix, a)
+    if kind == _lib.G_PHASE:
+        return phase_matrix(a)
+    if kind == _lib.G_EXPG:
+        q = g @ g
+        return np.eye(g.shape[0]) - q + np.cos(a) * q - 1j * np.sin(a) * g
+    raise ValueError("no host matrix for kind %d" % kind)
+
+
+def is_generator(m: np.ndarray, tol: float = 1e-9) -> bool:
+    """Hermitian with m^3 = m (eigenvalues in {-1, 0, 1})."""
+    m = np.asarray(m, dtype=np.complex128)
+    return bool(np.allclose(m, m.conj().T, atol=tol) and np.allclose(m @ m @ m, m, atol=tol))
+
+
 def is_involution(m: np.ndarray, tol: float = 1e-9) -> bool:
     return bool(np.allclose(m @ m, np.eye(m.shape[0]), atol=tol))
 
@@ -217,16 +292,56 @@ class _GateFactory:
             raise ValueError("unknown gate %s" % name)
         return Gate(_reshape2(np.asarray(m, dtype=np.complex128)), name)
 
+    def adjoint(self) -> "_AdjointFactory":
+        return _AdjointFactory(self)
+
     def __repr__(self) -> str:
         return self.n
 
 
+class _AdjointFactory:
+    """The adjoint of a gate factory (GateF.adjoint / GateVF.adjoint,
+    gates.py:264-271, 355-368): same parameters, conjugate-transposed
+    matrix, name ``<gate>d``."""
+
+    def __init__(self, base: Any):
+        self.base = base
+        self.n = str(getattr(base, "n", "unknowngate")) + "d"
+
+    def __call__(self, **kws: Any) -> "Gate":
+        m = matrix_of(self.base(**kws))
+        return Gate(_reshape2(m.conj().T), self.n)
+
+    def adjoint(self) -> Any:
+        return self.base
+
+    def __repr__(self) -> str:
+        return self.n
+
+
+def adjoint_factory(gatef: Any) -> Any:
+    if hasattr(gatef, "adjoint") and not isinstance(gatef, _GateFactory):
+        return gatef.adjoint()
+    return _AdjointFactory(gatef)
+
+
 def matrix_of(u: Any) -> np.ndarray:
-    """Square complex128 matrix of a Gate / tensor / array in [2]*2k or 2D layout."""
+    """Square complex128 matrix of a Gate / tensor / array in [2]*2k or 2D
+    layout.  Gate matrices are constants of the program: a tensor that
+    requires a gradient (or is traced by torch.func) raises instead of being
+    silently detached."""
     if isinstance(u, Gate):
         u = u.tensor
     if hasattr(u, "detach"):
-        u = u.detach().cpu().numpy()
+        if getattr(u, "requires_grad", False):
+            raise NotImplementedError(
+                "gradients with respect to gate matrices are not supported: the matrix is a constant of the "
+                "compiled program; parametrise the gate by angles (rx/ry/rz/rxx/ryy/rzz/phase/exp1/exp/u/r/"
+                "cu/cr/crx/cry/crz/orx/ory/orz/cphase/iswap)")
+        try:
+            u = u.detach().cpu().numpy()
+        except Exception as e:  # functorch-wrapped tensors cannot be read
+            raise NotImplementedError("a gate matrix traced by torch.func cannot be used as a constant") from e
     m = np.asarray(u, dtype=np.complex128)
     d = int(round(np.sqrt(m.size)))
     if d * d != m.size:

@@ -90,7 +90,12 @@ def json2qir(tcqasm: List[Dict[str, Any]]) -> List[Dict[str, Any]]:
         param = {k: json_to_tensor(v) for k, v in dict(d.get("parameters", {})).items()}
         param = {k: (v.item() if np.ndim(v) == 0 else v) for k, v in param.items()}
         name = d["name"]
-        gatef = getattr(G, name)
+        if hasattr(G, name):
+            gatef = getattr(G, name)
+        elif name.endswith("d") and hasattr(G, name[:-1]):
+            gatef = G.adjoint_factory(getattr(G, name[:-1]))  # an inverse()'s adjoint gate
+        else:
+            raise ValueError("unknown gate %r in JSON" % name)
         if "matrix" in d:
             gatem = G.Gate(G._reshape2(np.asarray(json_to_tensor(d["matrix"]), dtype=np.complex128)), name)
         elif "uparams" in d and name in _NEEDS_MATRIX:

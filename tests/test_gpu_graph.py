@@ -68,3 +68,35 @@ def test_captured_energy_grad_compiled(gpu):
         torch.cuda.synchronize()
         np.testing.assert_allclose(e1.cpu().numpy(), e0.cpu().numpy(), atol=1e-12)
         np.testing.assert_allclose(g1.cpu().numpy(), g0.cpu().numpy(), atol=1e-12)
+
+
+def test_captured_step_outlives_cache_eviction(gpu):
+    """A step captured through the Circuit API (cached programs and Pauli
+    sums) keeps replaying correctly after the engine's caches are emptied:
+    the step holds every object whose device tables the graph reads."""
+    import gc
+
+    import tensorcircuit_amd as tc
+    from tensorcircuit_amd import engine
+
+    n, L = 9, 2
+    p = torch.as_tensor(np.random.default_rng(4).uniform(0, 2 * np.pi, (2 * L, n)).astype(np.float32)).to(gpu)
+
+    def step():
+        c = tc.Circuit(n)
+        tc.templates.blocks.hea_layers(c, p, L)
+        return tc.backend.real(c.expectation_ps(z=[0, 1]) + c.expectation_ps(x=[3]))
+
+    g = engine.CapturedStep(step)
+    held = {type(o).__name__ for o in g.held}
+    assert "Program" in held and "PauliSum" in held
+    engine._PROG_CACHE.clear()
+    engine._PAULI_CACHE.clear()
+    gc.collect()
+    for rep in range(3):
+        p.add_(0.1)
+        out = g.replay().clone()
+        torch.cuda.synchronize()
+        ref = step()
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(out.cpu().numpy(), ref.cpu().numpy(), atol=1e-6)

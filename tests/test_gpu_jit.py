@@ -2,9 +2,12 @@
 straight-line generated code.  Checked against the complex128 oracle and
 against the generic fused kernels on the same inputs.
 
-Tolerances: complex64 states 4e-5 absolute (norm-1 states), energies and
-gradients 1e-4 absolute at complex64 and 1e-9 at complex128 (sums over up
-to a few hundred gates).
+Tolerances: complex64 states 4e-5 absolute (norm-1 states); energies and
+gradients 1e-5 absolute at complex64 (the target's max|dgrad| <= 1e-5) and
+1e-9 at complex128 (sums over up to a few hundred gates).  The 24-qubit
+workload the benchmark times (20 layers, checkpoints every 5, compiled
+passes, 5 register slots, searched schedule) is compared with its own
+complex128 run: max|dgrad| <= 1e-5 and |dE| <= 1e-5 |E|.
 """
 import numpy as np
 import pytest
@@ -98,9 +101,12 @@ def test_jit_energy_grad_random_vs_oracle(gpu, n, dtype, R):
     rdt = torch.float32 if dtype == "complex64" else torch.float64
     p = torch.as_tensor(params, dtype=rdt).reshape(1, -1).to(gpu)
     e, g = prog.energy_grad(p, ham, 1, dtype)
-    tol = 1e-4 if dtype == "complex64" else 1e-9
-    assert abs(float(np_of(e)[0]) - e_ref) < tol
-    np.testing.assert_allclose(np_of(g).reshape(-1), g_ref, atol=tol)
+    tol = 1e-5 if dtype == "complex64" else 1e-9
+    de = abs(float(np_of(e)[0]) - e_ref)
+    dg = float(np.max(np.abs(np_of(g).reshape(-1) - g_ref)))
+    print("n=%d %s R=%d |dE|=%.2e max|dgrad|=%.2e" % (n, dtype, R, de, dg))
+    assert de < tol
+    assert dg < tol
     st, _ = prog.forward(p, 1, dtype)
     ref = sv.run_ops(n, oops, params)
     np.testing.assert_allclose(np_of(st).reshape(-1), ref, atol=4e-5 if dtype == "complex64" else 1e-11)
@@ -132,8 +138,11 @@ def test_jit_matches_generic_hea_batch(gpu, R):
     s1, _ = jit.forward(pt, B, "complex64")
     np.testing.assert_allclose(np_of(s1), np_of(s0), atol=2e-5)
     e_ref, g_ref = sv.adjoint_value_and_grad(n, sv.hea_ops(n, L), ps[2].astype(np.float64).reshape(-1), terms)
-    assert abs(float(np_of(e1)[2]) - e_ref) < 1e-4
-    assert np.max(np.abs(np_of(g1)[2] - g_ref)) < 1e-4
+    de = abs(float(np_of(e1)[2]) - e_ref)
+    dg = float(np.max(np.abs(np_of(g1)[2] - g_ref)))
+    print("20q HEA R=%d |dE|=%.2e max|dgrad|=%.2e" % (R, de, dg))
+    assert de < 1e-5
+    assert dg < 1e-5
 
 
 def test_jit_vjp_matches_generic(gpu):
@@ -176,3 +185,34 @@ def test_jit_compile_reports_and_caches(gpu, tmp_path, monkeypatch):
     t2 = b.compile("complex64", 0)
     assert t1 > 0.0 and t2 == 0.0  # second build reads the cached code objects
     assert list((tmp_path / "jit").glob("*.co"))
+
+
+def test_c4_workload_vs_complex128(gpu):
+    """The benchmark's workload exactly (24 qubits, 20 HEA layers, TFIM
+    h = 1, checkpoints every 5 layers, compiled passes, default register
+    slots and searched schedule, parameter sets from default_rng(1004)):
+    the complex64 energies and gradients of two parameter sets against the
+    complex128 run of the same program (itself oracle-pinned at n <= 16 and
+    by the closed forms at 24 qubits)."""
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L, B, P = 24, 20, 32, 960
+    c = Circuit(n)
+    hea_layers(c, np.zeros((2 * L, n)), L)
+    specs, thetas = c._lower()
+    assert len(thetas) == P
+    terms = sv.tfim_terms(n)
+    ham = engine.PauliSum(n, np.array([t[1] for t in terms], dtype=np.int8), np.array([t[0] for t in terms]))
+    rng = np.random.default_rng(1004)
+    ps = rng.uniform(0, 2 * np.pi, size=(B, P)).astype(np.float32)[:2]
+    p64 = torch.as_tensor(ps).to(gpu)
+    prog = engine.Program(n, specs, P, fuse=1, ckpt_every=5, jit=1)
+    e, g = prog.energy_grad(p64, ham, 2, "complex64")
+    prog2 = engine.Program(n, specs, P, fuse=1, ckpt_every=5, jit=1)
+    e2, g2 = prog2.energy_grad(p64.double(), ham, 2, "complex128")
+    de = np.abs(np_of(e) - np_of(e2)) / np.abs(np_of(e2))
+    dg = np.max(np.abs(np_of(g) - np_of(g2)))
+    print("C4 24q L=20: rel|dE| %s max|dgrad| %.2e (|E| %s)" % (de, dg, np.abs(np_of(e2))))
+    assert np.all(de <= 1e-5)
+    assert dg <= 1e-5

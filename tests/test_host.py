@@ -208,7 +208,8 @@ def test_qir_round_trip():
     c.toffoli(0, 1, 2)
     d = Circuit.from_qir(c.to_qir())
     assert d._nqubits == 4 and len(d._ops) == len(c._ops)
-    for a, b in zip(c._ops, d._ops):
+    flat = lambda x: [o for e in x._ops for o in e]  # noqa: E731
+    for a, b in zip(flat(c), flat(d)):
         assert a.kind == b.kind and a.qubits == b.qubits and a.scale == b.scale
         assert float(a.theta or 0) == float(b.theta or 0)
         if a.mat is not None:
@@ -398,3 +399,203 @@ def test_default_register_slots():
     assert p20.describe("complex128", 0)["sched"]["R"] == 4
     assert prog(20, 2, jit=0).describe("complex64", 0)["sched"]["R"] == 4
     assert prog(16, 2, jit=1, reg_slots=5).describe("complex64", 0)["sched"]["R"] == 5
+
+
+def test_measurement_structures_lower_like_the_reference():
+    """any_measurements / any_local_measurements structure semantics
+    (templates/measurements.py:18-153): onehot=True takes one code per wire,
+    onehot=False a [n, 4] coefficient table over [I, X, Y, Z]; the docstring
+    example Z0 X2 lowers to one string either way."""
+    from tensorcircuit_amd.templates import measurements as M
+
+    low, shape = M._lowered(np.array([[0, 0, 0, 1], [1, 0, 0, 0], [0, 1, 0, 0]]), False, False, 3)
+    assert shape == () and low.ham.codes.tolist() == [[3, 0, 1]] and low.W.tolist() == [[1.0]]
+    low, shape = M._lowered(np.array([3, 0, 1]), True, False, 3)
+    assert shape == () and low.ham.codes.tolist() == [[3, 0, 1]]
+    # a stack of two structures: one output each, shared strings deduplicated
+    low, shape = M._lowered(np.array([[3, 0, 1], [3, 0, 1]]), True, False, 3)
+    assert shape == (2,) and low.ham.codes.shape[0] == 1 and low.W.tolist() == [[1.0], [1.0]]
+    # a general table multiplies out: (0.5 I + Z) x X x (Y + 2 Z) = 4 strings
+    low, shape = M._lowered(np.array([[0.5, 0, 0, 1], [0, 1, 0, 0], [0, 0, 1, 2]]), False, False, 3)
+    got = {tuple(c): low.W[0, t] for t, c in enumerate(low.ham.codes.tolist())}
+    assert got == {(0, 1, 2): 0.5, (0, 1, 3): 1.0, (3, 1, 2): 1.0, (3, 1, 3): 2.0}
+    # local: wire i alone; identity codes elsewhere
+    low, shape = M._lowered(np.array([3, 3, 1]), True, True, 3)
+    assert shape == (3,)
+    assert sorted(low.ham.codes.tolist()) == sorted([[3, 0, 0], [0, 3, 0], [0, 0, 1]])
+    with pytest.raises(ValueError):
+        M._lowered(np.array([3, 0]), True, False, 3)
+    with pytest.raises(ValueError):
+        M._lowered(np.array([3, 0, 4]), True, False, 3)
+    with pytest.raises(ValueError):  # 4^7 strings
+        M._lowered(np.ones((7, 4)), False, False, 7)
+
+
+def test_backend_transforms_semantics():
+    """grad / value_and_grad / vmap / vvag return shapes and values of
+    backends/pytorch_backend.py:497-709, checked against hand-derived
+    derivatives of a polynomial."""
+    import torch
+
+    from tensorcircuit_amd.backend import backend as K
+
+    def f(x, y):
+        return (x ** 2 * y).sum()
+
+    x = torch.tensor([1.0, 2.0])
+    y = torch.tensor(3.0)
+    v, g = K.value_and_grad(f)(x, y)
+    assert float(v) == 15.0 and g.tolist() == [6.0, 12.0]
+    v, (gx, gy) = K.value_and_grad(f, argnums=(0, 1))(x, y)
+    assert gx.tolist() == [6.0, 12.0] and float(gy) == 5.0
+    assert K.grad(f, argnums=1)(x, y).item() == 5.0
+    g, aux = K.grad(lambda x: ((x ** 2).sum(), x + 1), has_aux=True)(x)
+    assert g.tolist() == [2.0, 4.0] and isinstance(aux, tuple) and aux[0].tolist() == [2.0, 3.0]
+    (v, aux), g = K.value_and_grad(lambda x: ((x ** 2).sum(), x + 1), has_aux=True)(x)
+    assert float(v) == 5.0 and aux.tolist() == [2.0, 3.0] and g.tolist() == [2.0, 4.0]
+    xs = torch.tensor([[1.0, 2.0], [0.5, -1.0], [3.0, 0.0]])
+    assert K.vmap(lambda x, y: x * y)(xs, y).tolist() == (xs * 3).tolist()
+    assert K.vmap(lambda x, y: x * y, vectorized_argnums=(0, 1))(xs, xs).tolist() == (xs * xs).tolist()
+    vs, (gxs, gy) = K.vvag(f, argnums=(0, 1))(xs, y)
+    assert vs.tolist() == [(r ** 2).sum().item() * 3 for r in xs]
+    assert gxs.tolist() == (2 * xs * 3).tolist()          # vectorised: per sample
+    assert float(gy) == float((xs ** 2).sum())            # not vectorised: summed
+    vs, gy1 = K.vvag(f, argnums=1)(xs, y)
+    assert float(gy1) == float(gy)
+
+
+def _entry_matrix(ops, qubits):
+    """Host matrix of one QIR entry's lowered ops on the entry's qubits."""
+    from tensorcircuit_amd import _lib, gates as G
+
+    k = len(qubits)
+    m = np.eye(2 ** k, dtype=np.complex128)
+    for o in ops:
+        if o.kind == _lib.G_MAT:
+            f = np.asarray(o.mat)
+        else:
+            th = float(o.theta.detach().real) if hasattr(o.theta, "detach") else float(o.theta)
+            if o.kind == _lib.G_EXP1:
+                f = G.exp1_matrix(o.mat, o.scale * th)
+            else:
+                f = G.factor_matrix(o.kind, o.mat, o.scale * th)
+        pos = [qubits.index(q) for q in o.qubits]
+        full = sv.apply_gate  # embed f on positions pos of k qubits via the oracle
+        cols = [full(np.eye(2 ** k)[:, j].astype(np.complex128), f, pos, k) for j in range(2 ** k)]
+        m = np.stack(cols, axis=1) @ m
+    return m
+
+
+def test_trainable_lowering_matches_gate_matrices():
+    """Gates called with tensor parameters lower to trainable factors
+    (controlled rotations, cphase, iswap as exp(-i a G) with G^3 = G; u, cu,
+    r, cr as factor products); the product of the factors equals the
+    reference's gate matrix (the oracle's gate_matrix)."""
+    import torch
+
+    from tensorcircuit_amd import Circuit, _lib
+
+    rng = np.random.default_rng(9)
+    cases = [("crx", (0, 2)), ("cry", (2, 1)), ("crz", (1, 0)), ("orx", (0, 1)), ("ory", (1, 2)),
+             ("orz", (2, 0)), ("cphase", (1, 2)), ("iswap", (0, 2)), ("u", (1,)), ("cu", (2, 0)),
+             ("r", (0,)), ("cr", (1, 2))]
+    for name, qs in cases:
+        kw = {"theta": torch.tensor(float(rng.uniform(-3, 3)))}
+        if name in ("u", "cu"):
+            kw.update(phi=torch.tensor(float(rng.uniform(-3, 3))), lbd=float(rng.uniform(-3, 3)))
+        if name in ("r", "cr"):
+            kw.update(alpha=torch.tensor(float(rng.uniform(-3, 3))), phi=float(rng.uniform(-3, 3)))
+        c = Circuit(3)
+        getattr(c, name)(*qs, **kw)
+        ops = c._ops[0]
+        assert all(o.kind != _lib.G_MAT for o in ops), name
+        fkw = {k: float(v) for k, v in kw.items()}
+        if name == "cr":
+            ref = sv._ctrl(sv.r_gate(fkw["theta"], fkw["alpha"], fkw["phi"]))
+        else:
+            ref = sv.gate_matrix(name, **fkw)
+        np.testing.assert_allclose(_entry_matrix(ops, list(qs)), ref, atol=1e-12, err_msg=name)
+        # numbers keep the single host matrix
+        c2 = Circuit(3)
+        getattr(c2, name)(*qs, **fkw)
+        assert len(c2._ops[0]) == 1 and c2._ops[0][0].kind == _lib.G_MAT
+
+
+def test_host_gates_refuse_gradients():
+    """A requires_grad tensor reaching a host-evaluated constant raises
+    instead of being detached."""
+    import torch
+
+    from tensorcircuit_amd import Circuit, gates
+
+    th = torch.tensor(0.3, requires_grad=True)
+    c = Circuit(2)
+    with pytest.raises(NotImplementedError):
+        c.unitary(0, unitary=torch.eye(2, dtype=torch.complex128, requires_grad=True))
+    with pytest.raises(NotImplementedError):  # exp1 of a non-involution
+        c.exp1(0, 1, unitary=np.diag([1.0, 2.0, 3.0, 4.0]), theta=th)
+    with pytest.raises(NotImplementedError):  # a user gate factory is host-evaluated
+        Circuit.apply_general_variable_gate_delayed(lambda theta: gates.rx_matrix(float(theta)), "mygate")(
+            c, 0, theta=th)
+    c.crx(0, 1, theta=th)  # trainable: accepted
+    c.exp(0, 1, unitary=np.kron(np.diag([0.0, 1.0]), gates._x_matrix), theta=th)  # G^3 = G: trainable
+
+
+def test_inverse_rewrites_qir_like_the_reference():
+    """abstractcircuit.py:428-507: reverse order, sd/td -> s/t, rotations
+    theta -> -theta (iswap default 1), u/cu (phi, lbd) -> (pi - lbd,
+    pi - phi), r/cr theta -> -theta, any/exp1/wroot -> adjoint named
+    '<gate>d'; the ops of from_qir(inverse().to_qir()) equal the inverse's."""
+    from tensorcircuit_amd import Circuit, gates
+
+    c = Circuit(3)
+    c.h(0)
+    c.sd(1)
+    c.td(2)
+    c.wroot(0)
+    c.rx(1, theta=0.4)
+    c.iswap(0, 2)
+    c.u(1, theta=0.1, phi=0.2, lbd=0.3)
+    c.cu(0, 1, theta=0.5, lbd=2.0)
+    c.r(2, theta=0.7, alpha=0.3)
+    c.exp1(0, 1, unitary=gates._zz_matrix, theta=0.25)
+    c.unitary(2, unitary=gates._h_matrix @ gates._t_matrix)
+    inv = c.inverse()
+    qir = inv.to_qir()
+    names = [d["gatef"].n for d in qir]
+    assert names == ["anyd", "exp1d", "r", "cu", "u", "iswap", "rx", "wrootd", "t", "s", "h"]
+    # the local name is kept, as the reference passes d["name"] on
+    assert [d["name"] for d in qir] == ["any", "exp1", "r", "cu", "u", "iswap", "rx", "wroot", "td", "sd", "h"]
+    assert qir[2]["parameters"]["theta"] == -0.7 and qir[2]["parameters"]["alpha"] == 0.3
+    assert np.isclose(qir[3]["parameters"]["lbd"], np.pi) and np.isclose(qir[3]["parameters"]["phi"], np.pi - 2.0)
+    assert np.isclose(qir[4]["parameters"]["phi"], np.pi - 0.3) and np.isclose(qir[4]["parameters"]["lbd"], np.pi - 0.2)
+    assert qir[5]["parameters"]["theta"] == -1.0 and qir[6]["parameters"]["theta"] == -0.4
+    # the inverse's matrices multiply the forward ones to the identity, entry by entry
+    fwd = [(ops, d) for ops, d in zip(c._ops, c._qir)]
+    for (iops, idd), (fops, fd) in zip(inv._entries(), reversed(fwd)):
+        qs = list(fd["index"])
+        np.testing.assert_allclose(_entry_matrix(iops, qs) @ _entry_matrix(fops, qs), np.eye(2 ** len(qs)),
+                                   atol=1e-12, err_msg=fd["name"])
+    d = Circuit.from_qir(inv.to_qir())
+    for (a, _), (b, da) in zip(d._entries(), inv._entries()):
+        np.testing.assert_allclose(_entry_matrix(a, list(da["index"])), _entry_matrix(b, list(da["index"])),
+                                   atol=1e-12)
+    # JSON round trip of an inverse keeps the adjoint gates
+    e = Circuit.from_json(inv.to_json())
+    assert [q["gatef"].n for q in e.to_qir()] == names
+
+
+def test_capture_scope_collects_launch_objects():
+    """Objects used while a capture scope is open are collected for the
+    captured graph's lifetime (the bookkeeping behind CapturedStep)."""
+    from tensorcircuit_amd import engine
+
+    a, b = object(), object()
+    engine._hold(a)  # no scope open: nothing recorded, no error
+    engine._SCOPE.stack = [[]]
+    try:
+        engine._hold(a, None, b)
+        engine._hold(a)
+        assert engine._SCOPE.stack[-1] == [a, b, a]
+    finally:
+        engine._SCOPE.stack = []

@@ -378,6 +378,12 @@ def test_plain_torch_baseline_matches_oracle():
         lambda q: plain_torch.hea_tfim_energy(q, n, L, dtype=torch.complex128))(pt)
     np.testing.assert_allclose(float(e), e_ref, atol=1e-12)
     np.testing.assert_allclose(g.numpy().reshape(-1), g_ref, atol=1e-12)
+    # the per-layer rematerialised form (the C4 CPU baseline) is the same function
+    pr = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    er = plain_torch.hea_tfim_energy(pr, n, L, dtype=torch.complex128, remat=True)
+    er.backward()
+    np.testing.assert_allclose(float(er.detach()), e_ref, atol=1e-12)
+    np.testing.assert_allclose(pr.grad.numpy().reshape(-1), g_ref, atol=1e-12)
 
 
 def test_param_gate_closed_forms():
