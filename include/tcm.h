@@ -196,6 +196,19 @@ int tcm_expect_pauli_sum(const tcm_pauli* ham, int32_t dtype, const void* state,
                          int64_t batch, double* out, void* workspace,
                          int64_t ws_bytes, void* stream);
 
+/* The same result as tcm_expect_pauli_sum without a workspace and without
+ * any device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The partial sums live in a static
+ * device array of 8 MiB per GPU (held until the process ends); a call whose
+ * batch needs more (tcm_pauli_workspace_bytes > 8 MiB) returns
+ * TCM_E_WORKSPACE.  `ham` must already be prepared on the current device
+ * for `dtype` (one tcm_expect_pauli_sum call with it), else TCM_E_ARG.
+ * Calls that overlap on one GPU share the array and race. */
+int tcm_expect_pauli_sum_noalloc(const tcm_pauli* ham, int32_t dtype,
+                                 const void* state, int64_t batch, double* out,
+                                 void* stream);
+
 /* out_state = alpha_b * (H psi_b), alpha given per batch item on the device
  * (fp64 [batch], NULL = 1).  The vector-Jacobian product of expectation
  * (backward of the reference's autograd through the expectation network,

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtcm.so")
 
-SOURCES = ["kernels.hip", "program.cpp", "engine.cpp", "passgen.cpp", "jit.cpp", "kernels_aux.hip"]
+SOURCES = ["kernels.hip", "program.cpp", "engine.cpp", "passgen.cpp", "jit.cpp", "kernels_aux.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TCM_OFFLOAD_ARCH", "gfx950")
@@ -49,7 +49,7 @@ def build(verbose: bool = True) -> str:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 6)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 7)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + [

@@ -94,6 +94,7 @@ SIGNATURES = {
     "tcm_pauli_compile": (C.c_int, [_vp, _i32, C.POINTER(C.c_double)]),
     "tcm_pauli_workspace_bytes": (C.c_int, [_vp, _i32, _i64, C.POINTER(_i64)]),
     "tcm_expect_pauli_sum": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tcm_expect_pauli_sum_noalloc": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp]),
     "tcm_apply_pauli_sum": (C.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp]),
     "tcm_pauli_terms_workspace_bytes": (C.c_int, [_vp, _i64, C.POINTER(_i64)]),
     "tcm_expect_pauli_terms": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),

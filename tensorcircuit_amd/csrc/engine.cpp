@@ -331,9 +331,11 @@ int64_t pauli_ws(const tcm_pauli* H, int dt, int64_t batch) {
 
 template <typename T>
 int pauli_energy_apply(tcm_pauli* H, int dt, const void* psi, void* out, const double* alpha, int64_t batch,
-                       double* epart, bool energy, bool apply, hipStream_t st) {
-  PauliDev* pd = nullptr;
-  if (int rc = ensure_pauli(H, dt, &pd)) return rc;
+                       double* epart, bool energy, bool apply, hipStream_t st, PauliDev* ready = nullptr) {
+  // `ready`: device objects already checked present (no upload or load)
+  PauliDev* pd = ready;
+  if (!pd)
+    if (int rc = ensure_pauli(H, dt, &pd)) return rc;
   const DPTerm* td = (const DPTerm*)pd->terms;
   const int64_t estride = (int64_t)pauli_slots(H, dt) * 2;
   const PauliTileSched& S = H->ts[dt];
@@ -713,6 +715,63 @@ int forward_multi_impl(const tcm_multi_desc* descs, int ndesc, int dtype, int in
 }  // namespace
 
 // ===========================================================================
+// ===========================================================================
+// ---- expectation with caller-provided scratch, no allocation ---------------
+namespace tcm {
+// The launches of tcm_expect_pauli_sum with `scratch` (device, scratch_bytes)
+// as the partial-sum buffer, for a Pauli sum whose device tables (and
+// compiled passes) already exist on the current device; never uploads,
+// loads or allocates.
+// argument and size checks of tcm_expect_pauli_sum_noalloc (host only, made
+// before anything touches the device)
+int pauli_expect_check(const tcm_pauli* ham, int32_t dtype, const void* state, int64_t batch, double* out,
+                       int64_t scratch_bytes) {
+  if (!ham || !valid_dtype(dtype) || batch < 0 || (batch > 0 && (!state || !out)))
+    return set_error(TCM_E_ARG, "bad arguments to tcm_expect_pauli_sum_noalloc");
+  const int64_t need = pauli_ws(ham, dtype, batch);
+  if (need > scratch_bytes)
+    return set_error(TCM_E_WORKSPACE,
+                     "tcm_expect_pauli_sum_noalloc: %lld states need %lld bytes of partial sums, the static scratch "
+                     "holds %lld; split the batch",
+                     (long long)batch, (long long)need, (long long)scratch_bytes);
+  return 0;
+}
+
+int pauli_expect_prepared(const tcm_pauli* ham, int32_t dtype, const void* state, int64_t batch, double* out,
+                          void* scratch, int64_t scratch_bytes, void* stream) {
+  if (int rc = pauli_expect_check(ham, dtype, state, batch, out, scratch_bytes)) return rc;
+  if (!scratch) return set_error(TCM_E_ARG, "tcm_expect_pauli_sum_noalloc: no scratch");
+  if (batch == 0) return 0;
+  int dev = 0;
+  if (int rc = current_device(&dev)) return rc;
+  tcm_pauli* H = const_cast<tcm_pauli*>(ham);
+  PauliDev* pd = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(H->mu);
+    if ((int)H->dev.size() > dev) {
+      PauliDev& d = H->dev[dev];
+      const bool tables = d.terms && d.groups && (!H->ts[dtype].ok || d.tpasses[dtype]);
+      const bool kernels = !(H->jit[dtype].state == 1 && !H->jit[dtype].chunk.empty() && d.jfns[dtype].empty());
+      if (tables && kernels) pd = &d;
+    }
+  }
+  if (!pd)
+    return set_error(TCM_E_ARG,
+                     "tcm_expect_pauli_sum_noalloc: the Pauli sum is not prepared on device %d (make one "
+                     "tcm_expect_pauli_sum call with it for this dtype first)",
+                     dev);
+  hipStream_t st = (hipStream_t)stream;
+  double* epart = (double*)scratch;
+  int rc = dtype == TCM_C64 ? pauli_energy_apply<float>(H, dtype, state, nullptr, nullptr, batch, epart, true, false, st, pd)
+                            : pauli_energy_apply<double>(H, dtype, state, nullptr, nullptr, batch, epart, true, false, st, pd);
+  if (rc) return rc;
+  const int ns = pauli_slots(H, dtype);
+  launch_sum_slots(epart, (int64_t)ns * 2, ns, 2, out, batch, st);
+  HIP_TRY(hipGetLastError(), "expect launch");
+  return 0;
+}
+}  // namespace tcm
+
 extern "C" {
 
 int tcm_abi_version(void) { return TCM_ABI_VERSION; }
