@@ -416,12 +416,15 @@ def run_c5(args):
 
 def run_c3(args):
     """1024 parameter sets of a 16-qubit 8-layer HEA plus 256 circuits with
-    qubit counts uniform in 10..20 (depth 6), complex64: all states in
-    pass-synchronous multi-program launches (tile sizes autotuned per
-    program, or the defaults) or through every program's compiled passes in
-    sequence -- whichever whole step runs fastest -- then fp64 TFIM
-    energies; the step is captured as a HIP graph.  Seed 1003: qubit counts
-    first."""
+    qubit counts uniform in 10..20 (depth 6), complex64, then fp64 TFIM
+    energies of every state; seed 1003 (qubit counts first, then angles).
+    Staging: programs compiled into straight-line pass kernels; the larger
+    programs' fusion width and tile autotuned (forward of the real batch);
+    then a step-level choice among the programs' passes in sequence and the
+    compiled multi-program launches (pass r of every program in one kernel),
+    each with default and with tuned programs, every candidate captured as a
+    HIP graph and timed whole; the default (compiled passes in sequence,
+    default programs) is replaced only by a candidate faster by 3 %."""
     import numpy as np
     import torch
 
@@ -435,96 +438,107 @@ def run_c3(args):
     for i, n in enumerate(ns):
         groups.setdefault(int(n), []).append(i)
     t_stage = time.perf_counter()
-    items, hams = [], []
-
-    def program(n, L, batch, tuned):
-        c = tc.Circuit(n)
-        hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
-        specs, thetas = c._lower()
-        if args.tile:
-            k = min(args.tile, n)
-        else:
-            k = engine.autotune_tile(n, specs, len(thetas), batch) if tuned else 0
-        return engine.Program(n, specs, len(thetas), fuse=1, tile_bits=k), len(thetas)
-
-    def hamiltonian(n):
-        # compiled Pauli passes (one kernel per tile pass, compile time is
-        # part of staging)
-        h = engine.PauliSum(n, *workloads.tfim_terms(n), jit=1)
-        h.compile("complex64")
-        return h
 
     sizes = [(16, 8, 1024)] + [(n, 6, len(groups[n])) for n in sorted(groups)]
-    params, states = [], []
+    params, states, structs = [], [], []
     for n, L, B in sizes:
         P = 2 * L * n
         params.append(torch.as_tensor(rng.uniform(0, 2 * np.pi, size=(B, P)).astype(np.float32)).cuda())
         states.append(torch.empty(B, 1 << n, dtype=torch.complex64, device="cuda"))
+        c = tc.Circuit(n)
+        hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
+        structs.append(c._lower())
+
+    def hamiltonian(n):
+        h = engine.PauliSum(n, *workloads.tfim_terms(n), jit=1)
+        h.compile("complex64")
+        return h
+
     hams = [hamiltonian(n) for n, _, _ in sizes]
 
-    def make_step(items):
-        def step():
-            engine.forward_multi(items)
-            return [h.expect(st) for (_, _, st), h in zip(items, hams)]
-        return step
-
-    def make_seq_step(items):
-        # every program's compiled passes one after another (no multi launch)
-        def step():
-            for prog, pa, st in items:
-                prog.forward(pa, pa.shape[0], "complex64", out=st)
-            return [h.expect(st) for (_, _, st), h in zip(items, hams)]
-        return step
-
-    # step-level tuning: multi-program launches with per-program autotuned
-    # tiles or the default tiles, or the programs' compiled passes in
-    # sequence; the fastest whole step wins (and is then captured)
-    candidates = {}
-    for tuned in (False, True):
-        items = [(program(n, L, B, tuned)[0], pa, st) for (n, L, B), pa, st in zip(sizes, params, states)]
-        t, _ = _timed(make_step(items), 3, 1)
-        candidates["multi_tuned" if tuned else "multi_default"] = (t, items, make_step(items))
-    if not args.tile:
-        items = []
-        for (n, L, B), pa, st in zip(sizes, params, states):
-            c = tc.Circuit(n)
-            hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
-            specs, thetas = c._lower()
-            prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1, reg_slots=args.reg_slots)
+    def programs(opts_list):
+        out = []
+        for (n, L, B), (specs, thetas), o in zip(sizes, structs, opts_list):
+            prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1, **o)
             prog.compile("complex64", 0)
-            items.append((prog, pa, st))
-        t, _ = _timed(make_seq_step(items), 3, 1)
-        candidates["compiled_sequence"] = (t, items, make_seq_step(items))
-    choice = min(candidates, key=lambda k: candidates[k][0])
-    items, step = candidates[choice][1], candidates[choice][2]
+            out.append(prog)
+        return out
+
+    default_opts = [{} for _ in sizes]
+    if args.tile or args.reg_slots:
+        default_opts = [{"tile_bits": min(args.tile, n) if args.tile else 0, "reg_slots": args.reg_slots}
+                        for n, _, _ in sizes]
+    # fusion width and tile of the programs holding >= 5 % of the amplitudes
+    total_amps = sum(B << n for n, _, B in sizes)
+    tuned_opts, tune_ms = [], {}
+    for (n, L, B), (specs, thetas), d in zip(sizes, structs, default_opts):
+        if (B << n) >= 0.05 * total_amps and not (args.tile or args.reg_slots):
+            o, t = engine.autotune_program(n, specs, len(thetas), B, "complex64", jit=1)
+            tune_ms["%dq_x%d" % (n, B)] = {k: round(v, 4) for k, v in t.items()}
+            tuned_opts.append(o)
+        else:
+            tuned_opts.append(d)
+
+    def energies():
+        return [h.expect(st) for st, h in zip(states, hams)]
+
+    def seq_step(progs):
+        def step():
+            for prog, pa, st in zip(progs, params, states):
+                prog.forward(pa, pa.shape[0], "complex64", out=st)
+            return energies()
+        return step
+
+    def multi_step(progs):
+        plan = engine.MultiPlan(progs, "complex64", batches=[B for _, _, B in sizes])
+        items = list(zip(progs, params, states))
+
+        def step():
+            plan.forward(items)
+            return energies()
+        step.plan = plan
+        return step
+
+    variants = {"default": programs(default_opts)}
+    if tuned_opts != default_opts:
+        variants["tuned"] = programs(tuned_opts)
+    steps = {}
+    for vname, progs in variants.items():
+        steps["sequence_" + vname] = seq_step(progs)
+        steps["multi_" + vname] = multi_step(progs)
+    graphs = {name: engine.CapturedStep(fn) for name, fn in steps.items()}
+    choice, cand_ms = engine.tune_step({k: g.replay for k, g in graphs.items()}, "sequence_default")
     staging = time.perf_counter() - t_stage
 
-    dt_eager, es = _timed(step, args.steps, args.warmup)
-    # the same step captured once into a HIP graph and replayed (one launch
-    # per step instead of ~60 launches and their host-side argument set-up)
-    t_cap = time.perf_counter()
-    graph = engine.CapturedStep(step)
-    capture_s = time.perf_counter() - t_cap
-    dt, es = _timed(graph.replay, args.steps, args.warmup)
-    nstates = sum(st.shape[0] for _, _, st in items)
-    amps = sum(st.numel() for _, _, st in items)
-    # per-circuit check against single-program runs of two groups
-    prog, pb, st = items[-1]
-    s1, _ = prog.forward(pb, pb.shape[0], "complex64")
-    dmax = float((s1 - st).abs().max())
+    step = steps[choice]
+    dt_eager, _ = _timed(step, args.steps, args.warmup)
+    dt, es = _timed(graphs[choice].replay, args.steps, args.warmup)
+    # every state against a single-program run of its program (the first set)
+    progs = variants["tuned" if choice.endswith("tuned") else "default"]
+    dmax = 0.0
+    for prog, pa, st in zip(progs, params, states):
+        s1, _ = prog.forward(pa[:1], 1, "complex64")
+        dmax = max(dmax, float((s1[0] - st[0]).abs().max()))
+    nstates = sum(st.shape[0] for st in states)
+    amps = sum(st.numel() for st in states)
     out = {"metric": "C3 batched uneven HEA states+energies/s", "value": round(nstates / dt, 2),
            "unit": "states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "c64", "data": "synthetic",
            "config": {"workload": "hea_batched_uneven", "states": nstates, "amplitudes": amps,
-                      "programs": len(items), "qubit_counts": {str(k): len(v) for k, v in sorted(groups.items())},
-                      "seed": 1003, "staging_s": round(staging, 3), "graph_capture_s": round(capture_s, 3),
-                      "eager_ms_per_step": round(dt_eager * 1e3, 3), "hip_graph": True, "step_choice": choice,
-                      "step_candidates_ms": {k: round(v[0] * 1e3, 3) for k, v in candidates.items()},
-                      "tiles": {str(p.n): p.tile_bits for p, _, _ in items}},
-           "energy_16q_first": float(es[0][0].real), "max_abs_diff_vs_single_launch": dmax}
-    print("c3: %.2f ms for %d states (%d programs) as a HIP graph, %.2f ms eager"
-          % (dt * 1e3, nstates, len(items), dt_eager * 1e3), file=sys.stderr)
+                      "programs": len(sizes), "qubit_counts": {str(k): len(v) for k, v in sorted(groups.items())},
+                      "seed": 1003, "staging_s": round(staging, 3), "eager_ms_per_step": round(dt_eager * 1e3, 3),
+                      "hip_graph": True, "step_choice": choice,
+                      "step_candidates_ms": {k: round(v, 4) for k, v in cand_ms.items()},
+                      "program_tuning_ms": tune_ms,
+                      "tuned_options": {"%dq_x%d" % (n, B): engine.tune_key(o)
+                                        for (n, _, B), o in zip(sizes, tuned_opts) if o},
+                      "multi_plan_compile_s": round(sum(getattr(steps[k], "plan").compile_seconds
+                                                        for k in steps if k.startswith("multi")), 2)},
+           "energy_16q_first": float(es[0][0].real), "max_abs_diff_vs_single_program": dmax}
+    print("c3: %.3f ms for %d states (%d programs) as a HIP graph (%s), %.3f ms eager; candidates %s"
+          % (dt * 1e3, nstates, len(sizes), choice, dt_eager * 1e3, out["config"]["step_candidates_ms"]),
+          file=sys.stderr)
     print(json.dumps(out))
 
 

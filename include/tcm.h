@@ -366,6 +366,30 @@ int tcm_forward_multi(const tcm_multi_desc* descs, int32_t ndesc,
                       int32_t dtype, int32_t init_zero, void* workspace,
                       int64_t ws_bytes, void* stream);
 
+/* Compiled multi-program launches (no reference counterpart beyond the
+ * same vmap over structures): for a fixed list of programs, pass r of every
+ * program is generated into one kernel (straight-line code per program, a
+ * block running as many of a program's tiles side by side as its thread
+ * count allows) and compiled once (hiprtc, cached on disk).  A step of the
+ * reference's vmapped uneven circuits is then one launch per pass index
+ * instead of one per program and pass.  Passes that cannot be compiled
+ * (plain gates, matrix-core passes) run as single-program launches in the
+ * same order, as do passes whose tiles side by side would need more than
+ * 64 KiB of LDS per block.  `batches` (nullable): the expected batch of
+ * each program, weighting the choice of threads per block toward the
+ * programs with the most amplitudes.  At most 48 programs per plan;
+ * *seconds (nullable): compile time, 0 when cached. */
+typedef struct tcm_multi_plan tcm_multi_plan; /* opaque, caller-owned */
+int tcm_multi_plan_build(tcm_program* const* progs, const int64_t* batches,
+                         int32_t nprog, int32_t dtype, tcm_multi_plan** out,
+                         double* seconds);
+int tcm_multi_plan_free(tcm_multi_plan* plan);
+/* Forward of every program of the plan on its descriptor (descs[i].prog
+ * must be the plan's program i); workspace as tcm_multi_workspace_bytes. */
+int tcm_forward_multi_plan(tcm_multi_plan* plan, const tcm_multi_desc* descs,
+                           int32_t ndesc, int32_t init_zero, void* workspace,
+                           int64_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,6 +117,10 @@ SIGNATURES = {
     "tcm_adjoint_vjp": (C.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "tcm_multi_workspace_bytes": (C.c_int, [C.POINTER(TcmMultiDesc), _i32, _i32, C.POINTER(_i64)]),
     "tcm_forward_multi": (C.c_int, [C.POINTER(TcmMultiDesc), _i32, _i32, _i32, _vp, _i64, _vp]),
+    "tcm_multi_plan_build": (C.c_int, [C.POINTER(_vp), C.POINTER(_i64), _i32, _i32, C.POINTER(_vp),
+                                       C.POINTER(C.c_double)]),
+    "tcm_multi_plan_free": (C.c_int, [_vp]),
+    "tcm_forward_multi_plan": (C.c_int, [_vp, C.POINTER(TcmMultiDesc), _i32, _i32, _vp, _i64, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

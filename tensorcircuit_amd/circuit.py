@@ -51,6 +51,10 @@ def _float_of(x: Any, what: str) -> float:
     return float(np.real(x))
 
 
+# fused matrix-core programs of constant circuits, by structure
+_MFMA_CACHE: Dict[Any, Any] = {}
+
+
 def _any_tensor(vars: Dict[str, Any]) -> bool:
     return any(isinstance(v, torch.Tensor) for v in vars.values())
 
@@ -275,8 +279,63 @@ class Circuit:
     def _program(self, specs: List[GateSpec], nparams: int) -> engine.Program:
         return engine.get_program(self._nstate, specs, nparams, fuse=1)
 
+    def _constant_specs(self) -> Optional[List[GateSpec]]:
+        """Every gate as a constant matrix when no parameter is a tensor
+        (nothing to differentiate), else None."""
+        specs: List[GateSpec] = []
+        for op in (o for entry in self._ops for o in entry):
+            if op.kind == _lib.G_BARRIER:
+                continue
+            if op.kind == _lib.G_MAT:
+                specs.append(GateSpec(_lib.G_MAT, op.qubits, -1, 0.0, 1.0, op.mat))
+                continue
+            if isinstance(op.theta, torch.Tensor):
+                return None
+            a = op.scale * float(np.real(op.theta))
+            if op.kind == _lib.G_EXP1:
+                m = G.exp1_matrix(op.mat, a)
+            elif op.kind in (_lib.G_RXX, _lib.G_RYY, _lib.G_RZZ):
+                gen = {_lib.G_RXX: G._xx_matrix, _lib.G_RYY: G._yy_matrix, _lib.G_RZZ: G._zz_matrix}[op.kind]
+                m = G.exp1_matrix(gen, a, half=True)
+            else:
+                m = G.factor_matrix(op.kind, op.mat, a)
+            specs.append(GateSpec(_lib.G_MAT, op.qubits, -1, 0.0, 1.0, np.asarray(m, dtype=np.complex128)))
+        return specs
+
+    def _mfma_program(self) -> Optional[engine.Program]:
+        """A constant circuit on >= 20 qubits in complex64: its gates fused
+        into dense 5-qubit blocks applied on the FP32 matrix cores (LDS tile
+        passes of consecutive blocks), or None when the circuit does not
+        qualify (tensor parameters, fewer qubits, complex128, or fusion
+        would not reduce the number of state passes)."""
+        if not cons.mfma_for(self._nstate) or self._nstate < 5:
+            return None
+        specs = self._constant_specs()
+        if not specs:
+            return None
+        from . import fusion
+
+        key = ("mfma", self._nstate, tuple(s.key() for s in specs))
+        prog = _MFMA_CACHE.get(key)
+        if prog is None:
+            blocks = fusion.fuse(specs, self._nstate)
+            if 4 * len(blocks) > len(specs):
+                return None  # too few gates per block to pay for dense 32 x 32 blocks
+            prog = engine.Program(self._nstate, blocks, 0, fuse=1, mfma=2)
+            _MFMA_CACHE[key] = prog
+            while len(_MFMA_CACHE) > 16:
+                _MFMA_CACHE.pop(next(iter(_MFMA_CACHE)))
+        return prog
+
     def _run(self) -> torch.Tensor:
         dev = engine.device()
+        mp = self._mfma_program()
+        if mp is not None:
+            inp = None
+            if self.inputs is not None:
+                inp = self.inputs.to(device=dev).to(engine._CDT[cons.dtypestr]).reshape(1, -1)
+            st, _ = mp.forward(None, 1, cons.dtypestr, inputs=inp)
+            return st.reshape(-1)
         specs, thetas = self._lower()
         params = self._params(thetas, dev)
         prog = self._program(specs, len(thetas))

@@ -17,6 +17,26 @@ jit: Optional[bool] = {"0": False, "1": True}.get(os.environ.get("TCM_JIT", ""),
 JIT_MIN_QUBITS = 20
 
 
+# Constant circuits on the FP32 matrix cores: None = automatic (complex64,
+# at least MFMA_MIN_QUBITS qubits, no tensor-valued parameter: gates fused
+# into dense 5-qubit blocks applied as real GEMMs), True / False = always
+# (when the circuit qualifies) / never.  TCM_MFMA=0/1 sets it.
+mfma: Optional[bool] = {"0": False, "1": True}.get(os.environ.get("TCM_MFMA", ""), None)
+MFMA_MIN_QUBITS = 20
+
+
+def set_mfma(flag: Optional[bool]) -> None:
+    """Choose the matrix-core policy for constant circuits (None = by size)."""
+    global mfma
+    mfma = flag
+
+
+def mfma_for(nqubits: int) -> bool:
+    if dtypestr != "complex64" or mfma is False:
+        return False
+    return bool(mfma) or nqubits >= MFMA_MIN_QUBITS
+
+
 def set_jit(flag: Optional[bool]) -> None:
     """Choose the pass-compiler policy (None = by state size)."""
     global jit

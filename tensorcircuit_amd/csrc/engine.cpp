@@ -19,6 +19,20 @@ struct tcm_program {
   Program p;
 };
 
+// compiled multi-program forward launches (tcm_multi_plan_build)
+struct tcm_multi_plan {
+  std::vector<tcm_program*> progs;
+  int dtype = 0;
+  int maxp = 0;                          // most passes of any program
+  std::vector<tcm::MultiPassInfo> info;  // per pass index r
+  std::vector<int32_t> kid;              // per r: kernel index, -1 = none
+  std::vector<std::vector<char>> code;
+  std::vector<int32_t> chunk_of;
+  double seconds = 0.0;
+  std::mutex mu;
+  std::vector<std::vector<void*>> mods, fns;  // per device
+};
+
 struct PauliDev {
   void* terms = nullptr;
   void* flat = nullptr;    // DPTermF per term in the caller's order
@@ -200,6 +214,37 @@ int64_t forward_ws(const Program& P, int dtype, int64_t batch) {
   return align_up(batch * (int64_t)std::max(S.mtab_size, 1) * (int64_t)tsize(dtype));
 }
 
+// one forward pass pi (tiled, matrix-core or plain); ckout: checkpoint copy
+// written by a tiled pass (nullable)
+template <typename T>
+int run_one_pass(Program& P, const Sched& S, const DeviceTables& D, int pi, const void* mtab, void* state,
+                 int64_t batch, void* ckout, hipStream_t st) {
+  const int n = P.n;
+  const bool mfma = S.k > 0 && S.dpass[pi].mfma;
+  const bool tiled = S.k > 0 && S.dpass[pi].plain_op < 0 && !mfma;
+  if (mfma) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (!launch_mfma_tile(S.dpass[pi], (const DPass*)D.dpass, pi, (const DOp*)D.dop, (const int32_t*)D.ukbits,
+                            (const uint32_t*)D.runoff, D.ctab, state, n, batch, st))
+        return set_error(TCM_E_UNSUPPORTED, "matrix-core tile pass %d: %d blocks on %d tile bits not supported", pi,
+                         S.dpass[pi].nmop, S.dpass[pi].k);
+    } else
+      return set_error(TCM_E_UNSUPPORTED, "matrix-core passes are complex64 only");
+  } else if (tiled) {
+    if (!S.jit.empty() && S.jit[pi].kernel >= 0) {
+      if (int rc = jit_launch_fwd(S, D, pi, sizeof(T) == 8, mtab, D.ctab, state, batch, n, ckout, st)) return rc;
+    } else if (S.R == 4) {
+      launch_fused_fwd<T, 4>(S, D, pi, mtab, D.ctab, state, batch, n, ckout, st);
+    } else {
+      return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
+    }
+  } else {
+    const DOp& op = S.k > 0 ? S.dop[S.dpass[pi].plain_op] : S.dop[pi];
+    launch_plain_gate<T, false>(op, (const int32_t*)D.ukbits, mtab, S.mtab_size, D.ctab, state, n, batch, st);
+  }
+  return 0;
+}
+
 template <typename T>
 int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const void* mtab, void* state,
                        int64_t batch, void* ckpts, hipStream_t st) {
@@ -223,26 +268,7 @@ int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const 
           if (!ckout && tiled) ckout = ckpt_ptr(c);
           else extra.push_back(c);
         }
-    if (mfma) {
-      if constexpr (std::is_same<T, float>::value) {
-        if (!launch_mfma_tile(S.dpass[pi], (const DPass*)D.dpass, pi, (const DOp*)D.dop, (const int32_t*)D.ukbits,
-                              (const uint32_t*)D.runoff, D.ctab, state, n, batch, st))
-          return set_error(TCM_E_UNSUPPORTED, "matrix-core tile pass %d: %d blocks on %d tile bits not supported", pi,
-                           S.dpass[pi].nmop, S.dpass[pi].k);
-      } else
-        return set_error(TCM_E_UNSUPPORTED, "matrix-core passes are complex64 only");
-    } else if (tiled) {
-      if (!S.jit.empty() && S.jit[pi].kernel >= 0) {
-        if (int rc = jit_launch_fwd(S, D, pi, sizeof(T) == 8, mtab, D.ctab, state, batch, n, ckout, st)) return rc;
-      } else if (S.R == 4) {
-        launch_fused_fwd<T, 4>(S, D, pi, mtab, D.ctab, state, batch, n, ckout, st);
-      } else {
-        return set_error(TCM_E_UNSUPPORTED, "register slots R=%d not built", S.R);
-      }
-    } else {
-      const DOp& op = S.k > 0 ? S.dop[S.dpass[pi].plain_op] : S.dop[pi];
-      launch_plain_gate<T, false>(op, (const int32_t*)D.ukbits, mtab, S.mtab_size, D.ctab, state, n, batch, st);
-    }
+    if (int rc = run_one_pass<T>(P, S, D, pi, mtab, state, batch, ckout, st)) return rc;
     for (size_t c : extra) launch_copy<T>(ckpt_ptr(c), state, ck_stride, st);
   }
   return 0;
@@ -712,9 +738,98 @@ int forward_multi_impl(const tcm_multi_desc* descs, int ndesc, int dtype, int in
   return 0;
 }
 
+// kernel argument of tcm_multi_k<kid>: one job per program of the launch
+// (the layout of TcmMJob in the generated source)
+struct MJobArg {
+  const void* state;
+  long long sstride;
+  const void* mtab;
+  long long mstride, mbase, dbase;
+  const void* ctab;
+  unsigned long long block0, ntiles;
+};
+constexpr int kMaxPlanPrograms = 48;  // 48 x 72-byte jobs fit the kernel-argument segment
+
+int plan_load(tcm_multi_plan* M, std::vector<void*>** fns) {
+  int dev = 0;
+  if (int rc = current_device(&dev)) return rc;
+  std::lock_guard<std::mutex> lk(M->mu);
+  if ((int)M->fns.size() <= dev) {
+    M->fns.resize(dev + 1);
+    M->mods.resize(dev + 1);
+  }
+  if (M->fns[dev].empty() && !M->chunk_of.empty())
+    if (int rc = load_modules(M->code, M->chunk_of, "tcm_multi_k", M->mods[dev], M->fns[dev])) return rc;
+  *fns = &M->fns[dev];
+  return 0;
+}
+
+template <typename T>
+int forward_plan_impl(tcm_multi_plan* M, const tcm_multi_desc* descs, int ndesc, int dtype, int init_zero, void* ws,
+                      hipStream_t st) {
+  std::vector<void*>* fns = nullptr;
+  if (int rc = plan_load(M, &fns)) return rc;
+  std::vector<DeviceTables*> D(ndesc);
+  std::vector<void*> mtab(ndesc);
+  Carve cv(ws);
+  for (int i = 0; i < ndesc; ++i) {
+    Program& P = descs[i].prog->p;
+    if (int rc = ensure_tables(P, dtype, 0, &D[i])) return rc;
+    const Sched& S = P.sched[dtype][0];
+    mtab[i] = cv.take(descs[i].batch * (int64_t)std::max(S.mtab_size, 1) * sizeof(T));
+    if (descs[i].batch <= 0) continue;
+    launch_prep<T>(S, *D[i], D[i]->ctab, descs[i].params, descs[i].param_stride, mtab[i], descs[i].batch, st);
+    if (init_zero) launch_init_zero<T>(descs[i].state, P.n, descs[i].batch, st);
+  }
+  for (int r = 0; r < M->maxp; ++r) {
+    std::vector<char> in_kernel(ndesc, 0);
+    if (M->kid[r] >= 0) {
+      const tcm::MultiPassInfo& I = M->info[r];
+      std::vector<MJobArg> jobs(I.jobs.size());
+      unsigned long long blocks = 0;
+      for (size_t q = 0; q < I.jobs.size(); ++q) {
+        const tcm::MultiPassJob& J = I.jobs[q];
+        const tcm_multi_desc& d = descs[J.prog];
+        const Sched& S = d.prog->p.sched[dtype][0];
+        MJobArg& a = jobs[q];
+        a.state = d.state;
+        a.sstride = 1ll << d.prog->p.n;
+        a.mtab = mtab[J.prog];
+        a.mstride = S.mtab_size;
+        a.mbase = J.mbase;
+        a.dbase = J.dbase;
+        a.ctab = D[J.prog]->ctab;
+        a.block0 = blocks;
+        a.ntiles = (unsigned long long)std::max<int64_t>(d.batch, 0) << J.nblk_log;
+        // an empty batch keeps one (clamped) block out of the launch: its
+        // job gets no blocks and the search never selects it
+        blocks += (a.ntiles + J.per_block - 1) / J.per_block;
+        in_kernel[J.prog] = d.batch > 0;
+      }
+      if (blocks > 0) {
+        // the kernel's argument is the fixed-size job array, by value
+        void* args[] = {jobs.data()};
+        size_t asz = jobs.size() * sizeof(MJobArg);
+        void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, jobs.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &asz,
+                       HIP_LAUNCH_PARAM_END};
+        (void)args;
+        const hipError_t e = hipModuleLaunchKernel((hipFunction_t)(*fns)[M->kid[r]], (unsigned)blocks, 1, 1,
+                                                   (unsigned)I.ntb, 1, 1, 0, st, nullptr, cfg);
+        if (e != hipSuccess) return set_error((int)e, "hipModuleLaunchKernel(multi): %s", hipGetErrorString(e));
+      }
+    }
+    for (int i = 0; i < ndesc; ++i) {
+      Program& P = descs[i].prog->p;
+      const Sched& S = P.sched[dtype][0];
+      if (in_kernel[i] || descs[i].batch <= 0 || r >= (int)S.passes.size()) continue;
+      if (int rc = run_one_pass<T>(P, S, *D[i], r, mtab[i], descs[i].state, descs[i].batch, nullptr, st)) return rc;
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
-// ===========================================================================
 // ===========================================================================
 // ---- expectation with caller-provided scratch, no allocation ---------------
 namespace tcm {
@@ -1312,6 +1427,87 @@ int tcm_forward_multi(const tcm_multi_desc* descs, int32_t ndesc, int32_t dtype,
                             : forward_multi_impl<double>(descs, ndesc, dtype, init_zero, workspace, st);
   if (rc) return rc;
   HIP_TRY(hipGetLastError(), "forward_multi launch");
+  return 0;
+}
+
+int tcm_multi_plan_build(tcm_program* const* progs, const int64_t* batches, int32_t nprog, int32_t dtype,
+                         tcm_multi_plan** out, double* seconds) {
+  if (!out) return set_error(TCM_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (seconds) *seconds = 0.0;
+  if (!valid_dtype(dtype) || nprog < 1 || !progs) return set_error(TCM_E_ARG, "bad arguments to tcm_multi_plan_build");
+  if (nprog > kMaxPlanPrograms)
+    return set_error(TCM_E_ARG, "tcm_multi_plan_build: %d programs (at most %d per plan)", nprog, kMaxPlanPrograms);
+  std::vector<const Program*> ps;
+  std::vector<double> weight;
+  for (int i = 0; i < nprog; ++i) {
+    if (!progs[i]) return set_error(TCM_E_ARG, "program %d is NULL", i);
+    if (batches && batches[i] < 0) return set_error(TCM_E_ARG, "program %d: negative batch", i);
+    ps.push_back(&progs[i]->p);
+    weight.push_back(std::ldexp(batches ? (double)batches[i] : 1.0, progs[i]->p.n));
+  }
+  tcm_multi_plan* M = new tcm_multi_plan();
+  M->progs.assign(progs, progs + nprog);
+  M->dtype = dtype;
+  for (const Program* p : ps) M->maxp = std::max(M->maxp, (int)p->sched[dtype][0].passes.size());
+  M->info.resize(M->maxp);
+  M->kid.assign(M->maxp, -1);
+  std::vector<std::string> bodies;
+  for (int r = 0; r < M->maxp; ++r) {
+    std::string src = gen_multi_source(ps, weight, dtype, r, (int)bodies.size(), &M->info[r]);
+    if (src.empty()) continue;
+    if (M->info[r].lds > 160u * 1024u) {
+      const size_t lds = M->info[r].lds;
+      delete M;
+      return set_error(TCM_E_UNSUPPORTED, "multi-program pass %d needs %zu bytes of LDS per block", r, lds);
+    }
+    M->kid[r] = (int)bodies.size();
+    bodies.push_back(std::move(src));
+  }
+  if (!bodies.empty())
+    if (int rc = compile_bodies(gen_multi_prelude(dtype), bodies, "multi", M->code, M->chunk_of, &M->seconds)) {
+      delete M;
+      return rc;
+    }
+  if (seconds) *seconds = M->seconds;
+  *out = M;
+  return 0;
+}
+
+int tcm_multi_plan_free(tcm_multi_plan* plan) {
+  if (!plan) return 0;
+  int cur = -1;
+  const bool have = hipGetDevice(&cur) == hipSuccess;
+  for (size_t d = 0; d < plan->mods.size(); ++d) {
+    if (plan->mods[d].empty()) continue;
+    (void)hipSetDevice((int)d);
+    for (void* m : plan->mods[d])
+      if (m) (void)hipModuleUnload((hipModule_t)m);
+  }
+  if (have) (void)hipSetDevice(cur);
+  (void)hipGetLastError();
+  delete plan;
+  return 0;
+}
+
+int tcm_forward_multi_plan(tcm_multi_plan* plan, const tcm_multi_desc* descs, int32_t ndesc, int32_t init_zero,
+                           void* workspace, int64_t ws_bytes, void* stream) {
+  if (!plan || ndesc != (int)plan->progs.size() || !descs)
+    return set_error(TCM_E_ARG, "tcm_forward_multi_plan: %d descriptors for a plan of %zu programs", ndesc,
+                     plan ? plan->progs.size() : (size_t)0);
+  for (int i = 0; i < ndesc; ++i) {
+    const tcm_multi_desc& d = descs[i];
+    if (d.prog != plan->progs[i]) return set_error(TCM_E_ARG, "descriptor %d: not the plan's program %d", i, i);
+    if (d.batch < 0 || (d.batch > 0 && !d.state)) return set_error(TCM_E_ARG, "descriptor %d: bad arguments", i);
+    if (d.prog->p.nparams > 0 && d.batch > 0 && !d.params) return set_error(TCM_E_ARG, "descriptor %d: params required", i);
+  }
+  const int dtype = plan->dtype;
+  if (multi_ws(descs, ndesc, dtype) > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = dtype == TCM_C64 ? forward_plan_impl<float>(plan, descs, ndesc, dtype, init_zero, workspace, st)
+                            : forward_plan_impl<double>(plan, descs, ndesc, dtype, init_zero, workspace, st);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError(), "forward_multi_plan launch");
   return 0;
 }
 

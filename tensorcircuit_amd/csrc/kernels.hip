@@ -553,13 +553,13 @@ __device__ __forceinline__ T grad_member(const T* im, int gen, int s0, int s1) {
 // matrix preparation: per batch item, evaluate every recipe from params
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, const DFactor* __restrict__ fac,
-                       const T* __restrict__ params, int64_t pstride, const T* __restrict__ ct,
-                       T* __restrict__ mtab, int64_t mstride) {
+__global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, int first_dg, int dg_log,
+                       const DFactor* __restrict__ fac, const T* __restrict__ params, int64_t pstride,
+                       const T* __restrict__ ct, T* __restrict__ mtab, int64_t mstride) {
   const int64_t b = blockIdx.x;
   T* mt = mtab + b * mstride;
   const T* pr = params ? params + b * pstride : nullptr;
-  for (int i = threadIdx.x; i < nrec; i += blockDim.x) {
+  for (int i = threadIdx.x; i < first_dg; i += blockDim.x) {
     const DRecipe r = rec[i];
     if (r.kind < 0) continue;
     double a = r.param >= 0 ? r.scale * (double)pr[r.param] : r.theta;
@@ -655,12 +655,16 @@ __global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, const DFactor*
     }
   }
   __syncthreads();
-  // fused diagonals: products of member diagonals over the group index j
-  for (int i = 0; i < nrec; ++i) {
-    const DRecipe r = rec[i];
-    if (r.kind >= 0) continue;
+  // fused diagonals (recipes first_dg..nrec-1): products of member diagonals
+  // over the group index j; thread t takes recipe first_dg + (t >> dg_log)
+  // and entry t & (2^dg_log - 1), so all recipes are built side by side
+  const int ndg = nrec - first_dg;
+  for (int t = threadIdx.x; t < (ndg << dg_log); t += blockDim.x) {
+    const DRecipe r = rec[first_dg + (t >> dg_log)];
+    const int j = t & ((1 << dg_log) - 1);
+    if (r.kind >= 0 || j >= (1 << r.R)) continue;
     T* o = mt + r.out;
-    for (int j = threadIdx.x; j < (1 << r.R); j += blockDim.x) {
+    {
       double pr_ = 1.0, pi_ = 0.0;
       for (int f = 0; f < r.nfac; ++f) {
         const DFactor F = fac[r.goff + f];
@@ -1646,9 +1650,17 @@ template <typename T>
 void launch_prep(const Sched& S, const DeviceTables& D, const void* ctab, const void* params, int64_t pstride,
                  void* mtab, int64_t batch, hipStream_t st) {
   if (S.recipes.empty() || batch <= 0) return;
-  hipLaunchKernelGGL(k_prep<T>, dim3((unsigned)batch), dim3(256), 0, st, (const DRecipe*)D.recipes,
-                     (int)S.recipes.size(), (const DFactor*)D.factors, (const T*)params, pstride,
-                     (const T*)ctab, (T*)mtab, (int64_t)S.mtab_size);
+  // merged-diagonal (DG) recipes follow the base recipes; their widest group
+  const int nrec = (int)S.recipes.size();
+  int first_dg = nrec, dg_log = 0;
+  for (int i = 0; i < nrec; ++i)
+    if (S.recipes[i].kind < 0) {
+      if (first_dg == nrec) first_dg = i;
+      dg_log = std::max(dg_log, (int)S.recipes[i].R);
+    }
+  hipLaunchKernelGGL(k_prep<T>, dim3((unsigned)batch), dim3(256), 0, st, (const DRecipe*)D.recipes, nrec, first_dg,
+                     dg_log, (const DFactor*)D.factors, (const T*)params, pstride, (const T*)ctab, (T*)mtab,
+                     (int64_t)S.mtab_size);
 }
 
 template <typename T>

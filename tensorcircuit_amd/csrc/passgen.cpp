@@ -12,6 +12,7 @@
 // per-item matrix offsets) share one kernel: offsets into the per-item
 // table are emitted relative to the pass's smallest offset, which the
 // launcher passes as `mbase`.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -453,7 +454,12 @@ bool pass_supported(const Sched& S, const DPass& dp) {
 
 // body of one pass kernel (everything after the signature); mbase / dbase
 // are the smallest base-recipe / merged-diagonal table offsets it reads
-std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt, int* mbase_out, int* dbase_out) {
+// multi: the forward body as a device function of a multi-program launch:
+// block and thread indices come in as `bid` / `tid_` and the LDS tile as the
+// pointer `tile` (the kernel shares one LDS buffer among its programs'
+// bodies); every block is live (no early return: the caller clamps)
+std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt, int* mbase_out, int* dbase_out,
+                      bool multi = false) {
   const DPass& dp = S.dpass[pi];
   const HPass& hp = S.passes[pi];
   Gen G(S, dp, hp, adj);
@@ -502,10 +508,11 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
       if (a >> t & 1) o |= 1ull << hp.tile[t];
     return o;
   };
-  G.line("if ((unsigned long long)blockIdx.x >= (unsigned long long)total) return;");
-  G.line("const unsigned tid = threadIdx.x;");
-  G.line(fmt("const unsigned long long blk = blockIdx.x & 0x%llxull;", (nbl >= 64) ? ~0ull : ((1ull << nbl) - 1)));
-  G.line(fmt("const unsigned long long b = (unsigned long long)blockIdx.x >> %d;", nbl));
+  if (!multi) G.line("if ((unsigned long long)blockIdx.x >= (unsigned long long)total) return;");
+  G.line(multi ? "const unsigned tid = tid_;" : "const unsigned tid = threadIdx.x;");
+  const char* BI = multi ? "bid" : "(unsigned long long)blockIdx.x";
+  G.line(fmt("const unsigned long long blk = %s & 0x%llxull;", BI, (nbl >= 64) ? ~0ull : ((1ull << nbl) - 1)));
+  G.line(fmt("const unsigned long long b = %s >> %d;", BI, nbl));
   G.line(fmt("const unsigned long long base = %s;", deposit_expr("blk", bs, bd, "unsigned long long").c_str()));
   G.line(fmt("%s dlo = %s;", IT, deposit_expr("tid", ts, td, IT).c_str()));
   G.line("unsigned slo = swz(tid) * (unsigned)sizeof(cT);");
@@ -673,7 +680,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     return G.out;
   }
   // forward pass: one tile
-  G.line(fmt("__shared__ cT tile[%d];", 1 << k));
+  if (!multi) G.line(fmt("__shared__ cT tile[%d];", 1 << k));
   G.line("cT* __restrict__ s = st + b * sstride + base;");
   for (int it = 0; it < nit; ++it) G.line(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
   for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tile, slo ^ %uu) = r%d;", swz_h((uint32_t)it << tl) * cb, it));
@@ -768,6 +775,131 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
 }  // namespace tcm
 
 namespace tcm {
+
+// ---------------------------------------------------------------------------
+// Multi-program forward launches: pass r of every program of a plan in one
+// kernel.  Each program's pass body is a device function; a block of NTB
+// threads runs F = NTB / NT tiles of one program side by side (NT = that
+// pass's threads per tile), each with its own slice of the kernel's LDS,
+// all in lockstep through the same barriers.  Block -> program by the
+// job table (block0 prefix sums); the tile index of a sub-block past the
+// program's last tile is clamped to the last tile, which the same block
+// also computes, so the duplicate writes equal values after the last
+// barrier and no read races them.
+// ---------------------------------------------------------------------------
+constexpr size_t kMultiLds = 64u * 1024u;
+
+bool multi_pass_ok(const Sched& S, int r) {
+  return r < (int)S.dpass.size() && S.k > 0 && pass_supported(S, S.dpass[r]);
+}
+
+std::string gen_multi_prelude(int dt) {
+  return std::string("#define REAL ") + (dt == 0 ? "float" : "double") + "\n" + kPrelude +
+         "struct TcmMJob { const void* state; long long sstride; const void* mtab; long long mstride, mbase, "
+         "dbase; const void* ctab; unsigned long long block0, ntiles; };\n";
+}
+
+// the kernel `tcm_multi_k<kid>` for pass r (device functions and kernel; the
+// common prelude is gen_multi_prelude); empty when no program has a
+// compilable pass r
+std::string gen_multi_source(const std::vector<const Program*>& progs, const std::vector<double>& weight, int dt,
+                             int r, int kid, MultiPassInfo* info) {
+  info->jobs.clear();
+  info->ntb = 64;
+  info->lds = 0;
+  const size_t cb = dt == 0 ? 8 : 16;
+  std::vector<std::string> fns;
+  std::unordered_map<std::string, int> seen;
+  std::string calls;
+  // threads per block: the most any included pass needs; a pass joins only
+  // if its tiles side by side fit kMultiLds of LDS (two blocks per CU), so
+  // one wide pass does not lower the occupancy of the whole launch; the
+  // others run as single-program launches
+  auto joins = [&](const Sched& S) {
+    if (!multi_pass_ok(S, r)) return false;
+    const DPass& dp = S.dpass[r];
+    return (size_t)(info->ntb >> dp.tthreads_log) * (cb << dp.k) <= kMultiLds;
+  };
+  // the thread count that lets the most work (weight: amplitudes per
+  // program) join
+  {
+    double best = -1.0;
+    int best_ntb = 64;
+    for (int ntb = 64; ntb <= 1024; ntb *= 2) {
+      info->ntb = ntb;
+      double w = 0.0;
+      for (size_t i = 0; i < progs.size(); ++i) {
+        const Sched& S = progs[i]->sched[dt][0];
+        if (joins(S) && (1 << S.dpass[r].tthreads_log) <= ntb) w += weight[i];
+      }
+      if (w > best) {
+        best = w;
+        best_ntb = ntb;
+      }
+    }
+    info->ntb = best_ntb;
+  }
+  for (size_t i = 0; i < progs.size(); ++i) {
+    const Program& P = *progs[i];
+    const Sched& S = P.sched[dt][0];
+    if (!joins(S) || (1 << S.dpass[r].tthreads_log) > info->ntb) continue;
+    const DPass& dp = S.dpass[r];
+    int mb = 0, db = 0;
+    const std::string body = pass_body(P, S, r, false, dt, &mb, &db, true);
+    if (body.empty()) continue;
+    int id;
+    auto it = seen.find(body);
+    if (it == seen.end()) {
+      id = (int)fns.size();
+      seen.emplace(body, id);
+      fns.push_back(fmt("__device__ __forceinline__ void tcm_mb%d_%d(cT* __restrict__ st, long long sstride, "
+                        "const T* __restrict__ mtab, long long mstride, long long mbase, long long dbase, "
+                        "const T* __restrict__ ct, cT* __restrict__ ck, unsigned long long bid, unsigned tid_, "
+                        "cT* __restrict__ tile) {\n",
+                        kid, id) +
+                    body + "}\n");
+    } else {
+      id = it->second;
+    }
+    const int NT = 1 << dp.tthreads_log;
+    const int F = info->ntb / NT;
+    const size_t tile = cb << dp.k;
+    info->lds = std::max(info->lds, tile * (size_t)F);
+    MultiPassJob j{};
+    j.prog = (int)i;
+    j.nblk_log = dp.nblk_log;
+    j.per_block = F;
+    j.mbase = mb;
+    j.dbase = db;
+    const int q = (int)info->jobs.size();
+    info->jobs.push_back(j);
+    calls += fmt("    case %d: {\n"
+                 "      const unsigned sub = threadIdx.x / %du, t = threadIdx.x %% %du;\n"
+                 "      const TcmMJob& jb = J.job[%d];\n"
+                 "      unsigned long long tl = (gb - jb.block0) * %dull + sub;\n"
+                 "      if (tl >= jb.ntiles) tl = jb.ntiles - 1;\n"
+                 "      tcm_mb%d_%d((cT*)jb.state, jb.sstride, (const T*)jb.mtab, jb.mstride, jb.mbase, jb.dbase, "
+                 "(const T*)jb.ctab, nullptr, tl, t, reinterpret_cast<cT*>(lds + sub * %zuu));\n"
+                 "    } break;\n",
+                 q, NT, NT, q, F, kid, id, tile);
+  }
+  if (info->jobs.empty()) return std::string();
+  const int nj = (int)info->jobs.size();
+  std::string src = fmt("struct TcmMJobs%d { TcmMJob job[%d]; };\n", kid, nj);
+  for (const auto& f : fns) src += f;
+  const size_t lds_b = info->lds;
+  const int blocks_cu = (int)std::max<size_t>(1, (160u * 1024u) / std::max<size_t>(lds_b, 1));
+  const int waves = std::max(1, std::min(8, blocks_cu * (info->ntb / 64) / 4));
+  src += fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "
+             "tcm_multi_k%d(TcmMJobs%d J) {\n"
+             "  __shared__ __attribute__((aligned(16))) char lds[%zu];\n"
+             "  const unsigned long long gb = blockIdx.x;\n"
+             "  int q = 0;\n"
+             "  while (q + 1 < %d && gb >= J.job[q + 1].block0) ++q;\n"
+             "  switch (q) {\n%s    default: break;\n  }\n}\n",
+             info->ntb, waves, kid, kid, lds_b, nj, calls.c_str());
+  return src;
+}
 
 // ---------------------------------------------------------------------------
 // Pauli-sum tile passes, compiled: for every pass of the tiled Pauli schedule

@@ -226,6 +226,22 @@ struct Program {
 };
 
 // pass compiler (passgen.cpp, jit.cpp)
+// one program's part of a multi-program forward launch (passgen.cpp)
+struct MultiPassJob {
+  int32_t prog;       // index in the plan's program list
+  int32_t nblk_log;   // tiles per state = 2^nblk_log
+  int32_t per_block;  // tiles per block of the launch (sub-blocks)
+  int32_t mbase, dbase;
+};
+struct MultiPassInfo {
+  std::vector<MultiPassJob> jobs;
+  int32_t ntb = 64;   // threads per block of the launch
+  size_t lds = 0;     // LDS bytes per block
+};
+bool multi_pass_ok(const Sched& S, int r);
+std::string gen_multi_prelude(int dt);
+std::string gen_multi_source(const std::vector<const Program*>& progs, const std::vector<double>& weight, int dt,
+                             int r, int kid, MultiPassInfo* info);
 std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, std::vector<JitPass>& jp,
                             std::vector<std::string>& bodies);
 int jit_build(const Program& P, Sched& S, int dt, int mode);

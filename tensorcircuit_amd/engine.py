@@ -15,6 +15,7 @@ from __future__ import annotations
 import collections
 import ctypes as C
 import json
+import os
 import threading
 from typing import Any, Dict, List, Optional, Sequence, Tuple
 
@@ -252,8 +253,61 @@ def forward_multi(items: Sequence[Tuple[Program, Optional[torch.Tensor], torch.T
                                 _stream()), "tcm_forward_multi")
 
 
-_TUNE: Dict[str, int] = {}
+_TUNE: Dict[str, Any] = {}
 _TUNE_FILE = None
+
+
+class MultiPlan:
+    """Compiled multi-program forward launches for a fixed list of programs
+    (tcm_multi_plan_build): pass r of every program runs in one kernel, so
+    many small circuits share the GPU with large ones instead of running one
+    after another.  ``forward(items)`` takes (program, params, state) in the
+    plan's program order."""
+
+    def __init__(self, programs: Sequence[Program], dtype: str = "complex64",
+                 batches: Optional[Sequence[int]] = None):
+        """``batches``: the expected batch of each program (weights the
+        kernels' block shape toward the programs with the most amplitudes)."""
+        self.programs = list(programs)
+        self.dtype = dtype
+        arr = (C.c_void_p * len(self.programs))(*[p.handle.value for p in self.programs])
+        bt = None if batches is None else (C.c_int64 * len(self.programs))(*[int(b) for b in batches])
+        h = C.c_void_p()
+        sec = C.c_double(0.0)
+        check(lib.tcm_multi_plan_build(arr, bt, len(self.programs), _CODE[dtype], C.byref(h), C.byref(sec)),
+              "tcm_multi_plan_build")
+        self.handle = h
+        self.compile_seconds = sec.value
+
+    def __del__(self, _free=lib.tcm_multi_plan_free):
+        h = getattr(self, "handle", None)
+        if h:
+            _free(h)
+            self.handle = None
+
+    def forward(self, items: Sequence[Tuple[Program, Optional[torch.Tensor], torch.Tensor]],
+                init_zero: bool = True) -> None:
+        _hold(self, *self.programs)
+        dev = device()
+        code = _CODE[self.dtype]
+        descs = (_lib.TcmMultiDesc * max(len(items), 1))()
+        keep = []
+        for i, (prog, params, state) in enumerate(items):
+            b = state.shape[0]
+            p = None
+            if prog.nparams:
+                p = params.to(device=dev, dtype=_RDT[self.dtype]).reshape(b, prog.nparams).contiguous()
+                keep.append(p)
+            descs[i].prog = prog.handle.value
+            descs[i].params = _ptr(p)
+            descs[i].param_stride = prog.nparams
+            descs[i].state = state.data_ptr()
+            descs[i].batch = b
+        nb = _i64()
+        check(lib.tcm_multi_workspace_bytes(descs, len(items), code, C.byref(nb)), "workspace")
+        ws = _workspace(nb.value, dev)
+        check(lib.tcm_forward_multi_plan(self.handle, descs, len(items), 1 if init_zero else 0, ws.data_ptr(),
+                                         ws.numel(), _stream()), "tcm_forward_multi_plan")
 
 
 def _tune_path() -> str:
@@ -263,18 +317,49 @@ def _tune_path() -> str:
     return os.path.join(d, "autotune.json")
 
 
-def autotune_tile(n: int, specs: Sequence[GateSpec], nparams: int, batch: int, dtype: str = "complex64",
-                  candidates: Sequence[int] = (10, 11, 12, 13), reps: int = 3, **opts: Any) -> int:
-    """Pick the forward tile size (log2 amplitudes per LDS tile) by timing the
-    real program on the real batch; results are cached on disk keyed by
-    (n, dtype, batch, structure).  Staging cost, reported apart from running
-    time like the reference's jit staging (utils.py:205-232)."""
+def tune_key(opts: Dict[str, int]) -> str:
+    """Name of a (tile bits, register slots) choice; {} is "default"."""
+    if not opts:
+        return "default"
+    return "k%d_r%d" % (opts.get("tile_bits", 0), opts.get("reg_slots", 0))
+
+
+def _time_best(fn: Any, reps: int) -> float:
+    """Best of ``reps`` single runs in ms by HIP events on the current
+    stream (one slow run, e.g. a clock ramp, does not decide)."""
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    return best
+
+
+def autotune_program(n: int, specs: Sequence[GateSpec], nparams: int, batch: int, dtype: str = "complex64",
+                     tiles: Sequence[int] = (12, 13), slots: Sequence[int] = (4, 5), reps: int = 3,
+                     margin: float = 0.03, **opts: Any) -> Tuple[Dict[str, int], Dict[str, float]]:
+    """Pick the fusion width (register slots: qubits whose gates apply from
+    registers between LDS exchanges) and the tile (log2 amplitudes per LDS
+    tile) of a program by timing its forward on the real batch: the default
+    configuration and every (tile, slots) candidate, best of ``reps``.  A
+    candidate replaces the default only when faster by more than ``margin``
+    (so a tuned program is never slower than the default by this
+    measurement).  Returns (options for Program, ms per candidate); the
+    choice is cached on disk keyed by (n, dtype, batch, structure), and a
+    cached call returns an empty timing dict.  Staging cost, reported apart
+    from running time like the reference's jit staging
+    (utils.py:205-232)."""
     import hashlib
-    import os
 
     global _TUNE_FILE
+    jit = int(opts.get("jit", 0))
     h = hashlib.sha1(repr([s.key() for s in specs]).encode()).hexdigest()[:16]
-    key = "%d/%s/%d/%s" % (n, dtype, batch, h)
+    key = "prog/%d/%s/%d/%d/%s" % (n, dtype, batch, jit, h)
     if _TUNE_FILE is None:
         _TUNE_FILE = _tune_path()
         try:
@@ -283,30 +368,27 @@ def autotune_tile(n: int, specs: Sequence[GateSpec], nparams: int, batch: int, d
         except Exception:
             pass
     if key in _TUNE:
-        return _TUNE[key]
+        return dict(_TUNE[key]), {}
     dev = device()
-    rdt = _RDT[dtype]
-    p = torch.zeros(batch, max(nparams, 1), dtype=rdt, device=dev)[:, :nparams] if nparams else None
+    p = torch.zeros(batch, max(nparams, 1), dtype=_RDT[dtype], device=dev)[:, :nparams] if nparams else None
     st = torch.empty(batch, 1 << n, dtype=_CDT[dtype], device=dev)
-    best, best_t = 0, float("inf")
-    for k in candidates:
-        if k > n:
-            continue
-        prog = Program(n, specs, nparams, fuse=1, tile_bits=k, **opts)
-        prog.forward(p, batch, dtype, out=st)  # first call stages the tables
-        torch.cuda.synchronize()
-        # best of `reps` single runs (one slow run, e.g. a clock ramp, does
-        # not decide)
-        t = float("inf")
-        for _ in range(reps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            prog.forward(p, batch, dtype, out=st)
-            e1.record()
-            torch.cuda.synchronize()
-            t = min(t, e0.elapsed_time(e1))
-        if t < best_t:
-            best, best_t = k, t
+    cands: List[Dict[str, int]] = [{}]
+    for k in tiles:
+        for r in (slots if jit else (4,)):
+            if k <= n and k - r >= 6:
+                cands.append({"tile_bits": int(k), "reg_slots": int(r)})
+    times: Dict[str, float] = {}
+    for c in cands:
+        try:
+            prog = Program(n, specs, nparams, fuse=1, **dict(opts, **c))
+            if jit:
+                prog.compile(dtype, 0)
+            times[tune_key(c)] = _time_best(lambda: prog.forward(p, batch, dtype, out=st), reps)
+        except (ValueError, NotImplementedError):
+            continue  # a configuration this program cannot use
+    best = min(cands, key=lambda c: times.get(tune_key(c), float("inf")))
+    if times.get(tune_key(best), float("inf")) > times["default"] * (1.0 - margin):
+        best = {}
     _TUNE[key] = best
     try:
         os.makedirs(os.path.dirname(_TUNE_FILE), exist_ok=True)
@@ -314,7 +396,20 @@ def autotune_tile(n: int, specs: Sequence[GateSpec], nparams: int, batch: int, d
             json.dump(_TUNE, f)
     except Exception:
         pass
-    return best
+    return dict(best), times
+
+
+def tune_step(candidates: Dict[str, Any], default: str, reps: int = 3,
+              margin: float = 0.03) -> Tuple[str, Dict[str, float]]:
+    """Step-level choice: time each candidate step (a callable enqueuing a
+    whole step on the current stream), best of ``reps``, and return the
+    fastest; a candidate replaces ``default`` only when faster than it by
+    more than ``margin``.  Returns (name, ms per candidate)."""
+    times = {name: _time_best(fn, reps) for name, fn in candidates.items()}
+    best = min(times, key=times.get)
+    if times[best] > times[default] * (1.0 - margin):
+        best = default
+    return best, times
 
 
 _PROG_CACHE: "collections.OrderedDict[Tuple, Program]" = collections.OrderedDict()

@@ -28,10 +28,14 @@ MAX_STRINGS_PER_STRUCTURE = 4096
 
 
 def _host_table(structures: Any) -> np.ndarray:
+    """The table's values on the host.  Read with torch.func's transforms
+    off, so that a plain tensor used inside grad / vmap is not wrapped by the
+    read itself; a tensor batched by vmap raises RuntimeError (no storage)."""
     if isinstance(structures, torch.Tensor):
         if structures.requires_grad:
             raise ValueError("structure tables are data here: gradients with respect to them are not supported")
-        return structures.detach().cpu().numpy()
+        with torch._C._DisableFuncTorch():
+            return structures.detach().cpu().numpy()
     return np.asarray(structures)
 
 
