@@ -25,16 +25,17 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per adjoint pass launch measured with rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE on the default workload (24q, 20 layers, batch 32, checkpoints
-# every 5 layers, default tiles), FETCH scaled x2.0 and WRITE x1.0 by a 4 GiB
-# copy in the same run.  Key: (qubits, layers, batch, ckpt, jit, first-fit
+# every 5 layers, default tiles) over whole energy+grad steps
+# (--kernel-only eg, the timed work), FETCH scaled x2.0 and WRITE x1.0 by a
+# 4 GiB copy profiled the same way.  Key: (qubits, layers, batch, ckpt, jit, first-fit
 # scheduling).  Commands and per-counter figures:
 # profiles/tcm_adj_passes_hea24_b32.md (compiled passes, searched schedule)
 # and profiles/k_fused_adj_hea24_b32.md (generic kernels, first-fit schedule).
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 16484712844}
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 16719582208}
 # VALU wave-instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, mean over
 # the adjoint / forward pass launches of one sweep), same workload and key;
 # profiles/tcm_adj_passes_hea24_b32.md.
-MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (4252221440, 1579745280)}
+MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (3977347072, 1567424512)}
 # MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
 # every 4 cycles at 2.4 GHz
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
@@ -64,9 +65,10 @@ def parse():
     ap.add_argument("--config", choices=["c4", "c2", "c3", "c5"], default="c4",
                     help="c4 (default): 24q adjoint energy+grad; c2: 26q forward + energy; "
                          "c5: 30q fused-MFMA brickwork with a complex128 check")
-    ap.add_argument("--kernel-only", choices=["fwd", "adj", "copy", "sweep"], default=None,
-                    help="run only the fused forward passes, the adjoint sweep, a state copy of the same "
-                         "size, or the 1-qubit gate sweeps (for counter profiles); prints no JSON")
+    ap.add_argument("--kernel-only", choices=["eg", "fwd", "adj", "copy", "sweep"], default=None,
+                    help="run only whole energy+grad steps (the timed work), the fused forward passes, the "
+                         "adjoint sweep, a state copy of the same size, or the 1-qubit gate sweeps (for "
+                         "counter profiles); prints no JSON")
     return ap.parse_args()
 
 
@@ -127,6 +129,8 @@ def main():
                 prog.forward(params, B, "complex64", out=st)
             elif args.kernel_only == "adj":
                 prog.vjp(params, ck, st, st, B, "complex64", False)
+            elif args.kernel_only == "eg":
+                prog.energy_grad(params, ham, B)
             else:
                 torch.empty_like(st).copy_(st)
         torch.cuda.synchronize()
