@@ -574,6 +574,17 @@ __global__ void k_prep(const DRecipe* __restrict__ rec, int nrec, int first_dg, 
         double c, s;
         sincos(0.5 * a, &s, &c);
         o[0] = (T)c; o[1] = (T)-s; o[2] = (T)s; o[3] = (T)c;
+        // three-shear form (Paeth): the rotation by phi' is x -= t y,
+        // y += s' x, x -= t y with t = tan(phi' / 2), s' = sin phi'; phi' is
+        // phi = a / 2 reduced into [-pi/2, pi/2] (|t| <= 1) by
+        // R(phi) = -R(phi - pi), the sign kept in sigma
+        double ph = remainder(0.5 * a, 2.0 * M_PI);
+        double sg = 1.0;
+        if (fabs(ph) > 0.5 * M_PI) {
+          ph -= copysign(M_PI, ph);
+          sg = -1.0;
+        }
+        o[4] = (T)tan(0.5 * ph); o[5] = (T)sin(ph); o[6] = (T)sg; o[7] = 0;
       } break;
       case TCM_G_RZ: {
         double c, s;

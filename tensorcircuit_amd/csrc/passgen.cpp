@@ -68,10 +68,14 @@ __device__ __forceinline__ unsigned swz(unsigned t) {
 // LDS element at a byte offset (addresses are kept in bytes so the XOR
 // swizzle needs no per-access shift)
 #define LDSB(arr, off) (*reinterpret_cast<cT*>(reinterpret_cast<char*>(arr) + (off)))
+// sign bit of a table entry (1 for negative)
+__device__ __forceinline__ unsigned sgnb(float x) { return __float_as_uint(x) >> 31; }
+__device__ __forceinline__ unsigned sgnb(double x) { return (unsigned)((unsigned long long)__double_as_longlong(x) >> 63); }
 __device__ __forceinline__ T imcj(cT a, cT b) { return a.x * b.y - a.y * b.x; }
 __device__ __forceinline__ T recj(cT a, cT b) { return a.x * b.x + a.y * b.y; }
 union tcm_d2i { double d; int i[2]; };
-// row_shr moves inside each 16-lane row; lanes with no source read 0
+// DPP moves inside each 16-lane row (row_shr, row_ror, quad_perm); with
+// row_shr, lanes with no source read 0
 template <int CTRL>
 __device__ __forceinline__ double dpp_shr(double x) {
   tcm_d2i u, r;
@@ -115,6 +119,17 @@ template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_rows(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
 }
+// sum over each 16-lane row, in every lane of the row: quad_perm swaps of
+// lanes (i ^ 1, then i ^ 2) give each lane its quad's sum, row_ror 4 and 8
+// add the other three quads
+template <typename V>
+__device__ __forceinline__ V rowall(V x) {
+  x += dpp_shr<0xB1>(x);
+  x += dpp_shr<0x4E>(x);
+  x += dpp_shr<0x124>(x);
+  x += dpp_shr<0x128>(x);
+  return x;
+}
 // sum over the wave, total in lane 63 only: row prefix sums by
 // row_shr 1/2/4/8, then row_bcast:15 into rows 1 and 3 and row_bcast:31
 // into rows 2 and 3
@@ -144,6 +159,10 @@ struct Gen {
   int mbase, dbase;
   std::string out;
   std::vector<int> where;  // logical register index -> variable index
+  // sigma entries of the shear-form rotations applied so far in this pass:
+  // each applies R(phi') = sigma R(phi), so the pass output is exact once
+  // multiplied by the product of their sigmas (frame_fix)
+  std::vector<std::string> frame;
 
   Gen(const Sched& s, const DPass& d, const HPass& h, bool adj) : S(s), dp(d), hp(h), dag(adj), R(s.R) {
     nreg = 1 << R;
@@ -177,6 +196,31 @@ struct Gen {
                y.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(), pre.c_str(),
                pre.c_str()));
     }
+  }
+  // real rotation as three shears per real 2-vector (x, y) of a pair:
+  // x -= t y; y += s x; x -= t y (three fused multiply-adds instead of two
+  // multiplies and two multiply-adds)
+  void op_shear(const char* a, int S0, const std::string& pre) {
+    const int m = 1 << S0;
+    const char* P = pre.c_str();
+    for (int j = 0; j < nreg; ++j) {
+      if (j & m) continue;
+      const std::string x = var(a, j), y = var(a, j | m);
+      line(fmt("%s.x -= %st * %s.x; %s.y -= %st * %s.y;", x.c_str(), P, y.c_str(), x.c_str(), P, y.c_str()));
+      line(fmt("%s.x += %ss * %s.x; %s.y += %ss * %s.y;", y.c_str(), P, x.c_str(), y.c_str(), P, x.c_str()));
+      line(fmt("%s.x -= %st * %s.x; %s.y -= %st * %s.y;", x.c_str(), P, y.c_str(), x.c_str(), P, y.c_str()));
+    }
+  }
+  // negate every register of the arrays when the product of the pass's
+  // sigmas is -1 (sign bits XORed; block-uniform)
+  void frame_fix(const std::vector<const char*>& arrs) {
+    if (frame.empty()) return;
+    std::string e;
+    for (const auto& f : frame) e += fmt("%ssgnb(%s)", e.empty() ? "" : " ^ ", f.c_str());
+    line(fmt("if (%s) {", e.c_str()));
+    for (auto a : arrs)
+      for (int j = 0; j < nreg; ++j) line(fmt("  %s%d = cT{-%s%d.x, -%s%d.y};", a, j, a, j, a, j));
+    line("}");
   }
   void op_r1(const char* a, int S0, const std::string& pre) {
     const int m = 1 << S0;
@@ -260,8 +304,13 @@ struct Gen {
                    mel(op.moff, 6).c_str(), P, mel(op.moff, 7).c_str()));
         break;
       case OK_R1:
-        line(fmt("const T %sa = %s, %sb = %s, %sc = %s, %sd = %s;", P, mel(op.moff, 0).c_str(), P,
-                 mel(op.moff, dag ? 2 : 1).c_str(), P, mel(op.moff, dag ? 1 : 2).c_str(), P, mel(op.moff, 3).c_str()));
+        if (op.flags & OP_F_SHEAR)  // R(phi')^dagger = R(-phi'): t and s' change sign
+          line(fmt("const T %st = %s%s, %ss = %s%s;", P, dag ? "-" : "", mel(op.moff, 4).c_str(), P, dag ? "-" : "",
+                   mel(op.moff, 5).c_str()));
+        else
+          line(fmt("const T %sa = %s, %sb = %s, %sc = %s, %sd = %s;", P, mel(op.moff, 0).c_str(), P,
+                   mel(op.moff, dag ? 2 : 1).c_str(), P, mel(op.moff, dag ? 1 : 2).c_str(), P,
+                   mel(op.moff, 3).c_str()));
         break;
       case OK_D1:
         line(fmt("const T %sd0r = %s, %sd0i = %s%s, %sd1r = %s, %sd1i = %s%s;", P, mel(op.moff, 0).c_str(), P,
@@ -297,7 +346,14 @@ struct Gen {
     coeffs(op, pre);
     switch (op.kind) {
       case OK_U1: for (auto a : arrs) op_u1(a, op.s0, pre); break;
-      case OK_R1: for (auto a : arrs) op_r1(a, op.s0, pre); break;
+      case OK_R1:
+        if (op.flags & OP_F_SHEAR) {
+          for (auto a : arrs) op_shear(a, op.s0, pre);
+          frame.push_back(mel(op.moff, 6));
+        } else {
+          for (auto a : arrs) op_r1(a, op.s0, pre);
+        }
+        break;
       case OK_D1: for (auto a : arrs) op_d1(a, op.s0, pre); break;
       case OK_D2: for (auto a : arrs) op_d2(a, op.s0, op.s1, pre); break;
       case OK_DG: for (auto a : arrs) op_dg(a, pre); break;
@@ -429,12 +485,19 @@ struct Gen {
 // Gradient partials per trainable gate and block in an adjoint pass: one
 // per 16-lane row (fewest instructions) when that LDS fits as many blocks
 // per CU as one per wave does, else one per wave.
+// LDS slots of gradient partials: per row, gates are stored 16 at a time
+// (one lane each), so the count rounds up to a multiple of 16
+size_t red_slots(const DPass& dp, int per) {
+  const int NT = 1 << dp.tthreads_log;
+  const size_t g = (size_t)std::max(1, dp.ngop);
+  return (per == NT / 16 ? (g + 15) / 16 * 16 : g) * (size_t)per;
+}
+
 int adj_partials(int dt, const DPass& dp) {
   const int NT = 1 << dp.tthreads_log;
   const size_t tile = (size_t)(dt == 0 ? 8 : 16) << dp.k;
-  const size_t g = (size_t)std::max(1, dp.ngop);
   const size_t rb = dt == 0 ? 4 : 8;  // bytes per partial (working precision)
-  auto blocks = [&](int per) { return (160u * 1024u) / (tile + rb * g * (size_t)per); };
+  auto blocks = [&](int per) { return (160u * 1024u) / (tile + rb * red_slots(dp, per)); };
   return blocks(NT / 16) == blocks(NT / 64) ? NT / 16 : NT / 64;
 }
 
@@ -471,9 +534,6 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   // per wave (wsum63, lane 63)
   const int PR = adj ? adj_partials(dt, dp) : 1;
   const bool rows = PR == NT / 16;
-  const char* rsum = rows ? "rowsum" : "wsum63";
-  const uint32_t lmask = rows ? 15u : 63u;
-  const int lshift = rows ? 4 : 6;
   const char* IT = wide ? "unsigned long long" : "unsigned";
   int mb = -1, db = -1;
   for (int g = 0; g < dp.ngroups; ++g) {
@@ -522,6 +582,25 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   auto hi = [&](int it) { return (unsigned long long)dep_const((uint64_t)it << tl); };
 
   int opid = 0;
+  // gradient partials in emission order: slot e of `red` holds gate
+  // gl_of[e].  Per-row partials: every lane of a row gets the row sum
+  // (rowall), lane e % 16 keeps it in `gacc`, and each 16 gates are stored
+  // with one unpredicated LDS write per lane, so no exec-mask branch splits
+  // the straight-line gate code.  Per-wave partials: lane 63 stores.
+  std::vector<int> gl_of;
+  auto emit_grad = [&](const std::string& v, int gloc) {
+    const int e = (int)gl_of.size();
+    gl_of.push_back(gloc);
+    if (rows) {
+      const int j = e & 15;
+      if (j == 0) G.line(fmt("gacc = rowall(%s);", v.c_str()));
+      else G.line(fmt("{ const T gs = rowall(%s); gacc = lr == %du ? gs : gacc; }", v.c_str(), j));
+      if (j == 15) G.line(fmt("red[rw + %du] = gacc;", (e - 15) * PR));
+    } else {
+      G.line(fmt("{ const T gs = wsum63(%s); if ((tid & 63u) == 63u) red[%d + (tid >> 6)] = gs; }", v.c_str(),
+                 e * PR));
+    }
+  };
   // gradients and inverse gates of one group, in reverse program order, on
   // the register arrays p (psi) and l (lambda)
   auto emit_adj_ops = [&](const DGroup& dg) -> bool {
@@ -556,8 +635,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
           for (int q = 0; q < op.nmem; ++q) {
             const DMember& M = S.dmem[op.mem0 + q];
             if (M.gen == GN_Z) {
-              G.line(fmt("{ const T gs = %s(zs%d);", rsum, M.s0));
-              G.line(fmt("  if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs; }", lmask, lmask, M.gloc * PR, lshift));
+              emit_grad(fmt("zs%d", M.s0), M.gloc);
               continue;
             }
             std::string e;
@@ -569,18 +647,23 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
               if (sg) e += fmt(" %s im%d", sg > 0 ? "+" : "-", j);
             }
             if (e.empty()) e = " + 0";
-            G.line(fmt("{ const T ga = (T)0%s; const T gs = %s(ga);", e.c_str(), rsum));
-            G.line(fmt("  if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs; }", lmask, lmask, M.gloc * PR, lshift));
+            G.line("{");
+            G.line(fmt("const T ga = (T)0%s;", e.c_str()));
+            emit_grad("ga", M.gloc);
+            G.line("}");
           }
           G.line("}");
         } else if (op.gen != GN_NONE) {
           G.line("{");
           if (!G.grad(op.gen, op.s0, op.s1, op.goff)) return false;
-          G.line(fmt("const T gs = %s(ga);", rsum));
-          G.line(fmt("if ((tid & %uu) == %uu) red[%d + (tid >> %d)] = gs;", lmask, lmask, op.gloc * PR, lshift));
+          emit_grad("ga", op.gloc);
           G.line("}");
         }
         G.apply(op, {"p", "l"}, opid++);
+        // a gate's gradient reduction may overlap its own inverse, not the
+        // next gates: without this fence the scheduler stretches live
+        // ranges across gates and spills
+        G.line("__builtin_amdgcn_sched_barrier(0);");
       }
     return true;
   };
@@ -591,7 +674,12 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     G.line(fmt("__shared__ cT tb[%d];", 1 << k));
     // gradient partials: one per 16-lane row (or wave) and trainable gate, in
     // the working precision; the block sum below is fp64
-    G.line(fmt("__shared__ T red[%d];", std::max(1, dp.ngop) * PR));
+    G.line(fmt("__shared__ T red[%d];", (int)red_slots(dp, PR)));
+    if (rows) {
+      G.line("const unsigned lr = tid & 15u;");
+      G.line(fmt("const unsigned rw = lr * %du + (tid >> 4);", PR));
+      G.line("T gacc = 0;");
+    }
     {
       std::string decl = "cT";
       for (int j = 0; j < nreg; ++j) decl += fmt("%s p%d, l%d", j ? "," : "", j, j);
@@ -655,6 +743,9 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
       prev = g;
       prev_where = G.where;
     }
+    if ((int)gl_of.size() != dp.ngop) return std::string();
+    if (rows && (gl_of.size() & 15)) G.line(fmt("red[rw + %du] = gacc;", (int)(gl_of.size() & ~(size_t)15) * PR));
+    G.frame_fix({"p", "l"});
     G.line("asm volatile(\"\" : \"+v\"(dlo), \"+v\"(slo));");
     G.line("__syncthreads();");
     write_regs("p", prev, prev_where);
@@ -671,10 +762,13 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
       G.line(fmt("  so[dlo + %lluull] = LDSB(tb, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
     G.line("}");
     if (dp.ngop > 0) {
-      G.line(fmt("for (unsigned q = tid; q < %du; q += %du) {", dp.ngop, NT));
+      std::string tab;
+      for (int q : gl_of) tab += fmt("%s%d", tab.empty() ? "" : ", ", q);
+      G.line(fmt("static const unsigned short gl_of[%d] = {%s};", dp.ngop, tab.c_str()));
+      G.line(fmt("for (unsigned e = tid; e < %du; e += %du) {", dp.ngop, NT));
       G.line("  double sum = 0.0;");
-      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += (double)red[q * %d + w];", PR, PR));
-      G.line(fmt("  part[b * pstride + (long long)q * %lldll + (long long)blk] = sum;", 1ll << nbl));
+      G.line(fmt("  for (int w = 0; w < %d; ++w) sum += (double)red[e * %d + w];", PR, PR));
+      G.line(fmt("  part[b * pstride + (long long)gl_of[e] * %lldll + (long long)blk] = sum;", 1ll << nbl));
       G.line("}");
     }
     return G.out;
@@ -702,6 +796,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     G.line(fmt("const unsigned a0 = swz(%s) * (unsigned)sizeof(cT);", deposit_expr("tid", cs, cd, "unsigned").c_str()));
     for (int j = 0; j < nreg; ++j) G.line(fmt("cT v%d = LDSB(tile, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb));
     for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) G.apply(S.dop[o], {"v"}, opid++);
+    if (g == dp.ngroups - 1) G.frame_fix({"v"});
     G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
     for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tile, a1 ^ %uu) = v%d;", (uint32_t)dg.S[j] * cb, G.where[j]));
     G.line("}");
@@ -741,9 +836,7 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
       // register target: as many waves per SIMD as the LDS tiles allow
       // (160 KiB per CU), so registers never cut occupancy below that
       const size_t cbytes = dt == 0 ? 8 : 16;
-      const size_t lds = (cbytes << dp.k) +
-                         (mode == 0 ? 0 : (dt == 0 ? 4 : 8) * (size_t)std::max(1, dp.ngop) *
-                                              (size_t)adj_partials(dt, dp));
+      const size_t lds = (cbytes << dp.k) + (mode == 0 ? 0 : (dt == 0 ? 4 : 8) * red_slots(dp, adj_partials(dt, dp)));
       const int blocks_cu = (int)std::max<size_t>(1, (160u * 1024u) / lds);
       const int waves = std::max(1, std::min(8, blocks_cu * (NT / 64) / 4));
       // parameter lists: jit_launch_fwd / jit_launch_adj pass exactly these

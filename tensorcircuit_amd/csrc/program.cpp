@@ -638,7 +638,7 @@ static int lower(Program& P, const tcm_gate* gates, int32_t ngates) {
     }
     if (g.kind != TCM_G_MAT) {
       L.mslot = mslot;
-      mslot += layout_size(L.okind);
+      mslot += L.src == TCM_G_RY ? kRySlot : layout_size(L.okind);
       if (L.param >= 0) {
         L.gop = P.ngop++;
         L.gfac = 2.0 * c * L.scale;
@@ -691,7 +691,7 @@ static void base_recipes(const Program& P, std::vector<DRecipe>& out, int* size)
     r.theta = g.theta;
     r.scale = g.scale;
     out.push_back(r);
-    sz = std::max(sz, g.mslot + layout_size(g.okind));
+    sz = std::max(sz, g.mslot + (g.src == TCM_G_RY ? kRySlot : layout_size(g.okind)));
   }
   *size = sz;
 }
@@ -797,6 +797,7 @@ static void build_group_ops(const Program& P, Sched& S, const HGroup& hg,
     op.kind = g.okind;
     op.moff = g.mslot >= 0 ? g.mslot : (g.cslot >= 0 ? -g.cslot - 1 : 0);
     op.gen = g.gen;
+    if (g.src == TCM_G_RY && g.okind == OK_R1 && g.mslot >= 0) op.flags |= OP_F_SHEAR;
     op.goff = g.goff;
     op.gop = g.gop;
     op.gloc = g.gop >= 0 ? gloc_of_gop[g.gop] : -1;

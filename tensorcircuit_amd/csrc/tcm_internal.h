@@ -65,7 +65,9 @@ struct DOp {
 };
 
 // DOp flags
-enum : int32_t { OP_F_MFMA = 1 };  // dense 5-qubit op applied with FP32 MFMA
+// OP_F_SHEAR: an ry rotation whose table entry also holds the three-shear
+// form of its reduced angle (see k_prep): entries 4, 5, 6 = t, s, sigma
+enum : int32_t { OP_F_MFMA = 1, OP_F_SHEAR = 2 };
 
 // Gradient member of an OK_DG op (diagonal generator on slots of the group).
 struct DMember {
@@ -120,6 +122,11 @@ struct DRecipe {
   int32_t pad;
   double theta, scale;
 };
+
+// Table entries of an ry recipe: [c, -s, s, c] of the exact rotation by
+// phi = angle / 2, then [t, s', sigma, 0]: phi reduced to phi' in
+// [-pi/2, pi/2] with R(phi) = sigma R(phi'), t = tan(phi' / 2), s' = sin phi'
+constexpr int kRySlot = 8;
 
 // Factor of a DG recipe: a diagonal (D1 or D2 layout, per-batch or constant)
 // evaluated on slots s0 (, s1) of the group index.
