@@ -39,7 +39,14 @@ std::string fmt(const char* f, ...) {
   return out;
 }
 
-uint32_t swz_h(uint32_t t) { return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu); }
+// LDS swizzle of the compiled passes: the low 5 element bits XOR the higher
+// 5-bit fields, so that 32 lanes (256 bytes of complex64, all 64 banks) land
+// on distinct banks in more of the register-group exchanges than with the
+// 4-bit swizzle of the generic kernels (swz_host); linear over GF(2)
+uint32_t swz_h(uint32_t t) { return t ^ (((t >> 5) ^ (t >> 10) ^ (t >> 15)) & 0x1Fu); }
+// register offsets of a group: dg.S holds them in the generic kernels'
+// swizzle, which is its own inverse (it reads only bits it leaves alone)
+uint32_t sreg(int32_t s) { return swz_h(swz_host((uint32_t)s)); }
 
 // OR of bit fields: source bits src[i] -> destination bits dst[i], grouped
 // into runs of consecutive positions.
@@ -63,7 +70,7 @@ const char* kPrelude = R"(
 typedef REAL T;
 struct __attribute__((aligned(2 * sizeof(REAL)))) cT { T x, y; };
 __device__ __forceinline__ unsigned swz(unsigned t) {
-  return t ^ (((t >> 4) ^ (t >> 8) ^ (t >> 12) ^ (t >> 16)) & 0xFu);
+  return t ^ (((t >> 5) ^ (t >> 10) ^ (t >> 15)) & 0x1Fu);
 }
 // LDS element at a byte offset (addresses are kept in bytes so the XOR
 // swizzle needs no per-access shift)
@@ -711,11 +718,11 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     };
     auto write_regs = [&](const char* a, int g, const std::vector<int>& wh) {
       const DGroup& dg = S.dgroup[dp.group0 + g];
-      for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tb, ga%d ^ %uu) = %s%d;", g, (uint32_t)dg.S[j] * cb, a, wh[j]));
+      for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tb, ga%d ^ %uu) = %s%d;", g, sreg(dg.S[j]) * cb, a, wh[j]));
     };
     auto read_regs = [&](const char* a, int g) {
       const DGroup& dg = S.dgroup[dp.group0 + g];
-      for (int j = 0; j < nreg; ++j) G.line(fmt("%s%d = LDSB(tb, ga%d ^ %uu);", a, j, g, (uint32_t)dg.S[j] * cb));
+      for (int j = 0; j < nreg; ++j) G.line(fmt("%s%d = LDSB(tb, ga%d ^ %uu);", a, j, g, sreg(dg.S[j]) * cb));
     };
     for (int gg = 0; gg < ng1; ++gg) {
       const int g = ng1 - 1 - gg;
@@ -794,11 +801,11 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     }
     G.line("{");
     G.line(fmt("const unsigned a0 = swz(%s) * (unsigned)sizeof(cT);", deposit_expr("tid", cs, cd, "unsigned").c_str()));
-    for (int j = 0; j < nreg; ++j) G.line(fmt("cT v%d = LDSB(tile, a0 ^ %uu);", j, (uint32_t)dg.S[j] * cb));
+    for (int j = 0; j < nreg; ++j) G.line(fmt("cT v%d = LDSB(tile, a0 ^ %uu);", j, sreg(dg.S[j]) * cb));
     for (int o = dg.op0; o < dg.op0 + dg.nops; ++o) G.apply(S.dop[o], {"v"}, opid++);
     if (g == dp.ngroups - 1) G.frame_fix({"v"});
     G.line("unsigned a1 = a0; asm volatile(\"\" : \"+v\"(a1));");
-    for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tile, a1 ^ %uu) = v%d;", (uint32_t)dg.S[j] * cb, G.where[j]));
+    for (int j = 0; j < nreg; ++j) G.line(fmt("LDSB(tile, a1 ^ %uu) = v%d;", sreg(dg.S[j]) * cb, G.where[j]));
     G.line("}");
     G.line("__syncthreads();");
   }
