@@ -31,11 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # scheduling).  Commands and per-counter figures:
 # profiles/tcm_adj_passes_hea24_b32.md (compiled passes, searched schedule)
 # and profiles/k_fused_adj_hea24_b32.md (generic kernels, first-fit schedule).
-MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 16719582208}
+MEASURED_ADJ_TRAFFIC = {(24, 20, 32, 5, 0, 1): 17046845037, (24, 20, 32, 5, 1, 0): 16715319666}
 # VALU wave-instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU, mean over
 # the adjoint / forward pass launches of one sweep), same workload and key;
 # profiles/tcm_adj_passes_hea24_b32.md.
-MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (3977347072, 1567424512)}
+MEASURED_VALU_INSTS = {(24, 20, 32, 5, 1, 0): (3492724736, 1338048512)}
 # MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
 # every 4 cycles at 2.4 GHz
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
@@ -304,7 +304,7 @@ def main():
 def _cpu_baseline(n, L):
     """The reference's plain method (gate-by-gate tensordot, torch autograd,
     complex64) on the host cores.  The 12-qubit 4-layer case is timed in full
-    (median of 5).  A 24-qubit 20-layer eval takes about a minute, so it is
+    (median of 5).  A 24-qubit 20-layer eval takes about two minutes, so it is
     sampled: energy + gradient at 1 and at 3 layers (each layer under
     per-layer rematerialisation), twice each after a discarded warm-up;
     20 layers = t(1) + 19 (t(3) - t(1)) / 2 per pair, and the two pairs give
