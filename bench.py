@@ -564,7 +564,8 @@ def run_c2(args):
     ham = engine.PauliSum(n, codes, w)
     p = torch.as_tensor(workloads.hea_params(1002, L, n).reshape(1, -1)).cuda()
     t_stage = time.perf_counter()
-    prog = engine.Program(n, specs, len(thetas), fuse=1, jit=args.jit, reg_slots=args.reg_slots)
+    prog = engine.Program(n, specs, len(thetas), fuse=1, jit=args.jit, reg_slots=args.reg_slots,
+                          tile_bits=args.tile)
     if args.jit:
         prog.compile("complex64", 0)
         prog.compile("complex128", 0)
@@ -583,7 +584,8 @@ def run_c2(args):
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
            "data": "synthetic", "config": {"workload": "hea_tfim_forward", "nqubits": n, "layers": L,
-                                           "fwd_passes": prog.npasses, "seed": 1002, "jit": args.jit,
+                                           "fwd_passes": prog.npasses, "tile_bits": prog.tile_bits,
+                                           "seed": 1002, "jit": args.jit,
                                            "staging_s": round(staging, 2)},
            "energy_c64": float(e32[0].real), "energy_c128": float(e64[0].real),
            "abs_denergy": abs(float(e32[0].real) - float(e64[0].real)), "fidelity_vs_c128": fid}

@@ -1147,6 +1147,11 @@ int build_program(const tcm_gate* gates, int32_t ngates, int32_t n, int32_t npar
     // instead of 56 + 68)
     int kf = o.tile_bits > 0 ? o.tile_bits : (dt == 0 ? 12 : 11) + (o.jit ? 1 : 0);
     int ka = o.adj_tile_bits > 0 ? o.adj_tile_bits : (o.jit ? (dt == 0 ? 13 : 12) : (dt == 0 ? 11 : 10));
+    // a tile must fit one CU's 160 KiB of LDS with room for the adjoint's
+    // partial sums: 2^14 complex64 or 2^13 complex128 amplitudes (128 KiB)
+    const int kmax = dt == 0 ? 14 : 13;
+    kf = std::min(kf, kmax);
+    ka = std::min(ka, kmax);
     // fused passes need at least one full wave (64 threads) per tile
     bool fused = o.fuse != 0 && n >= R + 6;
     for (int mode = 0; mode < 2; ++mode) {
