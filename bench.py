@@ -424,11 +424,13 @@ def run_c3(args):
     energies of every state; seed 1003 (qubit counts first, then angles).
     Staging: programs compiled into straight-line pass kernels; the larger
     programs' fusion width and tile autotuned (forward of the real batch);
-    then a step-level choice among the programs' passes in sequence and the
-    compiled multi-program launches (pass r of every program in one kernel),
-    each with default and with tuned programs, every candidate captured as a
-    HIP graph and timed whole; the default (compiled passes in sequence,
-    default programs) is replaced only by a candidate faster by 3 %."""
+    then a step-level choice among the programs' passes in sequence, the
+    compiled multi-program launches (pass r of every program in one kernel)
+    and a hybrid (the programs holding >= 5 % of the amplitudes on their own
+    passes, the rest in one multi-program launch per pass), each with default
+    and with tuned programs, every candidate captured as a HIP graph and
+    timed whole; the default (compiled passes in sequence, default programs)
+    is replaced only by a candidate faster by 3 %."""
     import numpy as np
     import torch
 
@@ -503,6 +505,24 @@ def run_c3(args):
         step.plan = plan
         return step
 
+    # the programs holding >= 5 % of the amplitudes on their own passes, the
+    # small (launch-bound) ones sharing one multi-program launch per pass
+    large = [(B << n) >= 0.05 * total_amps for n, _, B in sizes]
+
+    def hybrid_step(progs):
+        small = [i for i, big in enumerate(large) if not big]
+        plan = engine.MultiPlan([progs[i] for i in small], "complex64", batches=[sizes[i][2] for i in small])
+        items = [(progs[i], params[i], states[i]) for i in small]
+        alone = [(progs[i], params[i], states[i]) for i, big in enumerate(large) if big]
+
+        def step():
+            for prog, pa, st in alone:
+                prog.forward(pa, pa.shape[0], "complex64", out=st)
+            plan.forward(items)
+            return energies()
+        step.plan = plan
+        return step
+
     variants = {"default": programs(default_opts)}
     if tuned_opts != default_opts:
         variants["tuned"] = programs(tuned_opts)
@@ -510,6 +530,8 @@ def run_c3(args):
     for vname, progs in variants.items():
         steps["sequence_" + vname] = seq_step(progs)
         steps["multi_" + vname] = multi_step(progs)
+        if any(large) and not all(large):
+            steps["hybrid_" + vname] = hybrid_step(progs)
     graphs = {name: engine.CapturedStep(fn) for name, fn in steps.items()}
     choice, cand_ms = engine.tune_step({k: g.replay for k, g in graphs.items()}, "sequence_default")
     staging = time.perf_counter() - t_stage
@@ -537,8 +559,8 @@ def run_c3(args):
                       "program_tuning_ms": tune_ms,
                       "tuned_options": {"%dq_x%d" % (n, B): engine.tune_key(o)
                                         for (n, _, B), o in zip(sizes, tuned_opts) if o},
-                      "multi_plan_compile_s": round(sum(getattr(steps[k], "plan").compile_seconds
-                                                        for k in steps if k.startswith("multi")), 2)},
+                      "multi_plan_compile_s": round(sum(steps[k].plan.compile_seconds
+                                                        for k in steps if hasattr(steps[k], "plan")), 2)},
            "energy_16q_first": float(es[0][0].real), "max_abs_diff_vs_single_program": dmax}
     print("c3: %.3f ms for %d states (%d programs) as a HIP graph (%s), %.3f ms eager; candidates %s"
           % (dt * 1e3, nstates, len(sizes), choice, dt_eager * 1e3, out["config"]["step_candidates_ms"]),
