@@ -145,6 +145,48 @@ def test_jit_matches_generic_hea_batch(gpu, R):
     assert dg < 1e-5
 
 
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_jit_ry_shear_angle_edges(gpu, dtype):
+    """ry runs in the compiled passes as three shears of the angle reduced
+    into [-pi/2, pi/2], with the dropped sign restored at the end of each
+    pass.  Angles at and around the reduction's edges (ry angle +-pi and
+    +-2 pi, i.e. a half-angle of +-pi/2 and +-pi), negative and large ones:
+    states, energies and gradients of a 12-qubit HEA against the oracle."""
+    from tensorcircuit_amd import Circuit, engine
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    n, L, B = 12, 4, 3
+    pi = np.pi
+    edges = [0.0, pi, -pi, 2 * pi, -2 * pi, 3 * pi, -3 * pi, pi + 1e-6, pi - 1e-6, 2 * pi - 1e-6, 4 * pi + 0.1,
+             -7.5, 20.0, -20.0, 1e-7, -1e-7, 0.5 * pi, 1.5 * pi, 2.5 * pi, -0.5 * pi]
+    rng = np.random.default_rng(31)
+    ps = np.empty((B, 2 * L, n))
+    ps[0] = np.resize(np.array(edges), (2 * L, n))
+    ps[1] = rng.uniform(-20, 20, size=(2 * L, n))
+    ps[2] = rng.choice(np.array(edges), size=(2 * L, n)) + rng.uniform(-1e-3, 1e-3, size=(2 * L, n))
+    rdt = np.float32 if dtype == "complex64" else np.float64
+    ps = ps.astype(rdt)
+    c = Circuit(n)
+    hea_layers(c, np.zeros((2 * L, n)), L)
+    specs, thetas = c._lower()
+    terms = sv.tfim_terms(n)
+    ham = engine.PauliSum(n, np.array([t[1] for t in terms], dtype=np.int8), np.array([t[0] for t in terms]))
+    prog = engine.Program(n, specs, len(thetas), fuse=1, ckpt_every=2, jit=1, reg_slots=5)
+    pt = torch.as_tensor(ps.reshape(B, -1)).to(gpu)
+    e, g = prog.energy_grad(pt, ham, B, dtype)
+    st, _ = prog.forward(pt, B, dtype)
+    tol = 1e-5 if dtype == "complex64" else 1e-9
+    for b in range(B):
+        p64 = ps[b].astype(np.float64).reshape(-1)
+        e_ref, g_ref = sv.adjoint_value_and_grad(n, sv.hea_ops(n, L), p64, terms)
+        de = abs(float(np_of(e)[b]) - e_ref)
+        dg = float(np.max(np.abs(np_of(g)[b] - g_ref)))
+        print("set %d %s |dE|=%.2e max|dgrad|=%.2e" % (b, dtype, de, dg))
+        assert de < tol and dg < tol
+        ref = sv.run_ops(n, sv.hea_ops(n, L), p64)
+        np.testing.assert_allclose(np_of(st)[b], ref, atol=4e-6 if dtype == "complex64" else 1e-12)
+
+
 def test_jit_vjp_matches_generic(gpu):
     """Vector-Jacobian product of the forward map (the autograd backward)."""
     from tensorcircuit_amd import Circuit, engine
