@@ -421,8 +421,9 @@ def run_c5(args):
 def run_c3(args):
     """1024 parameter sets of a 16-qubit 8-layer HEA plus 256 circuits with
     qubit counts uniform in 10..20 (depth 6), complex64, then fp64 TFIM
-    energies of every state; seed 1003 (qubit counts first, then angles).
-    Staging: programs compiled into straight-line pass kernels; the larger
+    energies of every state; seed 1003 (qubit counts first, then angles),
+    run through engine.UnevenBatch.  Staging: programs compiled into
+    straight-line pass kernels; the larger
     programs' fusion width and tile autotuned (forward of the real batch);
     then a step-level choice among the programs' passes in sequence, the
     compiled multi-program launches (pass r of every program in one kernel)
@@ -447,11 +448,10 @@ def run_c3(args):
     t_stage = time.perf_counter()
 
     sizes = [(16, 8, 1024)] + [(n, 6, len(groups[n])) for n in sorted(groups)]
-    params, states, structs = [], [], []
+    params, structs = [], []
     for n, L, B in sizes:
         P = 2 * L * n
         params.append(torch.as_tensor(rng.uniform(0, 2 * np.pi, size=(B, P)).astype(np.float32)).cuda())
-        states.append(torch.empty(B, 1 << n, dtype=torch.complex64, device="cuda"))
         c = tc.Circuit(n)
         hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
         structs.append(c._lower())
@@ -462,116 +462,22 @@ def run_c3(args):
         return h
 
     hams = [hamiltonian(n) for n, _, _ in sizes]
-
-    def programs(opts_list):
-        out = []
-        for (n, L, B), (specs, thetas), o in zip(sizes, structs, opts_list):
-            prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1, **o)
-            prog.compile("complex64", 0)
-            out.append(prog)
-        return out
-
-    default_opts = [{} for _ in sizes]
+    options = None
     if args.tile or args.reg_slots:
-        default_opts = [{"tile_bits": min(args.tile, n) if args.tile else 0, "reg_slots": args.reg_slots}
-                        for n, _, _ in sizes]
-    # fusion width and tile of the programs holding >= 5 % of the amplitudes
-    total_amps = sum(B << n for n, _, B in sizes)
-    tuned_opts, tune_ms = [], {}
-    for (n, L, B), (specs, thetas), d in zip(sizes, structs, default_opts):
-        if (B << n) >= 0.05 * total_amps and not (args.tile or args.reg_slots):
-            o, t = engine.autotune_program(n, specs, len(thetas), B, "complex64", jit=1)
-            tune_ms["%dq_x%d" % (n, B)] = {k: round(v, 4) for k, v in t.items()}
-            tuned_opts.append(o)
-        else:
-            tuned_opts.append(d)
-
-    def energies():
-        return [h.expect(st) for st, h in zip(states, hams)]
-
-    def seq_step(progs):
-        def step():
-            for prog, pa, st in zip(progs, params, states):
-                prog.forward(pa, pa.shape[0], "complex64", out=st)
-            return energies()
-        return step
-
-    def multi_step(progs):
-        plan = engine.MultiPlan(progs, "complex64", batches=[B for _, _, B in sizes])
-        items = list(zip(progs, params, states))
-
-        def step():
-            plan.forward(items)
-            return energies()
-        step.plan = plan
-        return step
-
-    # the programs holding >= 5 % of the amplitudes on their own passes, the
-    # small (launch-bound) ones sharing one multi-program launch per pass
-    large = [(B << n) >= 0.05 * total_amps for n, _, B in sizes]
-
-    def hybrid_step(progs):
-        small = [i for i, big in enumerate(large) if not big]
-        plan = engine.MultiPlan([progs[i] for i in small], "complex64", batches=[sizes[i][2] for i in small])
-        items = [(progs[i], params[i], states[i]) for i in small]
-        alone = [(progs[i], params[i], states[i]) for i, big in enumerate(large) if big]
-
-        def step():
-            for prog, pa, st in alone:
-                prog.forward(pa, pa.shape[0], "complex64", out=st)
-            plan.forward(items)
-            return energies()
-        step.plan = plan
-        return step
-
-    # the same split on two streams (default priority) joined at the end:
-    # the small programs' launches and energies run beside the large
-    # programs' passes instead of after them
-    side = torch.cuda.Stream()
-
-    def hybrid_streams_step(progs):
-        base = hybrid_step(progs)
-        small = [i for i, big in enumerate(large) if not big]
-        items = [(progs[i], params[i], states[i]) for i in small]
-
-        def step():
-            cur = torch.cuda.current_stream()
-            side.wait_stream(cur)
-            res = [None] * len(sizes)
-            with torch.cuda.stream(side):
-                base.plan.forward(items)
-                for i in small:
-                    res[i] = hams[i].expect(states[i])
-            for i, big in enumerate(large):
-                if big:
-                    progs[i].forward(params[i], params[i].shape[0], "complex64", out=states[i])
-                    res[i] = hams[i].expect(states[i])
-            cur.wait_stream(side)
-            return res
-        step.plan = base.plan
-        return step
-
-    variants = {"default": programs(default_opts)}
-    if tuned_opts != default_opts:
-        variants["tuned"] = programs(tuned_opts)
-    steps = {}
-    for vname, progs in variants.items():
-        steps["sequence_" + vname] = seq_step(progs)
-        steps["multi_" + vname] = multi_step(progs)
-        if any(large) and not all(large):
-            steps["hybrid_" + vname] = hybrid_step(progs)
-            steps["hybrid_streams_" + vname] = hybrid_streams_step(progs)
-    graphs = {name: engine.CapturedStep(fn) for name, fn in steps.items()}
-    choice, cand_ms = engine.tune_step({k: g.replay for k, g in graphs.items()}, "sequence_default")
+        options = [{"tile_bits": min(args.tile, n) if args.tile else 0, "reg_slots": args.reg_slots}
+                   for n, _, _ in sizes]
+    ub = engine.UnevenBatch([(n, specs, len(thetas)) for (n, _, _), (specs, thetas) in zip(sizes, structs)],
+                            [B for _, _, B in sizes], hams=hams, params=params, options=options,
+                            tune=not (args.tile or args.reg_slots))
+    states = ub.states
+    choice = ub.choice
     staging = time.perf_counter() - t_stage
 
-    step = steps[choice]
-    dt_eager, _ = _timed(step, args.steps, args.warmup)
-    dt, es = _timed(graphs[choice].replay, args.steps, args.warmup)
+    dt_eager, _ = _timed(ub.run_eager, args.steps, args.warmup)
+    dt, es = _timed(ub.run, args.steps, args.warmup)
     # every state against a single-program run of its program (the first set)
-    progs = variants["tuned" if choice.endswith("tuned") else "default"]
     dmax = 0.0
-    for prog, pa, st in zip(progs, params, states):
+    for prog, pa, st in zip(ub.programs, params, states):
         s1, _ = prog.forward(pa[:1], 1, "complex64")
         dmax = max(dmax, float((s1[0] - st[0]).abs().max()))
     nstates = sum(st.shape[0] for st in states)
@@ -584,12 +490,12 @@ def run_c3(args):
                       "programs": len(sizes), "qubit_counts": {str(k): len(v) for k, v in sorted(groups.items())},
                       "seed": 1003, "staging_s": round(staging, 3), "eager_ms_per_step": round(dt_eager * 1e3, 3),
                       "hip_graph": True, "step_choice": choice,
-                      "step_candidates_ms": {k: round(v, 4) for k, v in cand_ms.items()},
-                      "program_tuning_ms": tune_ms,
+                      "step_candidates_ms": {k: round(v, 4) for k, v in ub.candidate_ms.items()},
+                      "program_tuning_ms": {k: {c: round(v, 4) for c, v in t.items()}
+                                            for k, t in ub.tuning_ms.items()},
                       "tuned_options": {"%dq_x%d" % (n, B): engine.tune_key(o)
-                                        for (n, _, B), o in zip(sizes, tuned_opts) if o},
-                      "multi_plan_compile_s": round(sum(steps[k].plan.compile_seconds
-                                                        for k in steps if hasattr(steps[k], "plan")), 2)},
+                                        for (n, _, B), o in zip(sizes, ub.tuned_options) if o},
+                      "multi_plan_compile_s": round(ub.multi_plan.compile_seconds, 2) if ub.multi_plan else 0.0},
            "energy_16q_first": float(es[0][0].real), "max_abs_diff_vs_single_program": dmax}
     print("c3: %.3f ms for %d states (%d programs) as a HIP graph (%s), %.3f ms eager; candidates %s"
           % (dt * 1e3, nstates, len(sizes), choice, dt_eager * 1e3, out["config"]["step_candidates_ms"]),

@@ -1164,3 +1164,150 @@ class CapturedStep:
     def replay(self):
         self.graph.replay()
         return self.out
+
+
+class UnevenBatch:
+    """Many circuit structures (different gates or qubit counts), each with a
+    batch of parameter sets, run forward -- with a Pauli-sum energy per state
+    where a Hamiltonian is given -- as one step replayed from a HIP graph: the
+    reference's vmap over circuit structures (backends/pytorch_backend.py:
+    605-617) without one launch per structure and gate.
+
+    Staging: every structure is compiled into pass kernels; those holding at
+    least ``large_frac`` of all amplitudes get their fusion width (register
+    slots) and tile bits autotuned on their real batch (autotune_program).
+    Then whole candidate steps are captured as HIP graphs and timed
+    (tune_step): the programs' passes in sequence; one compiled
+    multi-program launch per pass for all (MultiPlan); the hybrid (large
+    programs on their own passes, the small, launch-bound ones in one
+    multi-program launch per pass); and the hybrid with the small part on a
+    second stream (default priority) joined at the end of the step -- each
+    with the default and with the tuned programs.  The default (passes in
+    sequence, default programs) is kept unless a candidate is faster by
+    ``margin``, so the chosen step is never slower by its own measurement.
+
+    ``structures[i]`` = (n, specs, nparams).  ``params[i]`` ([batch_i,
+    nparams_i], the dtype's real type) are inputs: update them in place
+    between runs.  ``states[i]`` ([batch_i, 2^n_i]) are the outputs.
+    ``run()`` replays the chosen step and returns the energies (a complex128
+    [batch_i] tensor per structure, None where no Hamiltonian was given),
+    overwritten by each run."""
+
+    def __init__(self, structures: Sequence[Tuple[int, Sequence[GateSpec], int]], batches: Sequence[int],
+                 hams: Optional[Sequence[Optional["PauliSum"]]] = None, dtype: str = "complex64",
+                 params: Optional[Sequence[torch.Tensor]] = None,
+                 options: Optional[Sequence[Dict[str, int]]] = None, tune: bool = True,
+                 large_frac: float = 0.05, margin: float = 0.03, reps: int = 3):
+        dev = device()
+        k = len(structures)
+        if len(batches) != k or k == 0:
+            raise ValueError("one batch size per structure (at least one structure)")
+        if dtype not in _CODE:
+            raise ValueError("dtype must be complex64 or complex128")
+        rdt = torch.float32 if dtype == "complex64" else torch.float64
+        cdt = torch.complex64 if dtype == "complex64" else torch.complex128
+        self.dtype = dtype
+        self.sizes = [(int(n), int(b)) for (n, _, _), b in zip(structures, batches)]
+        if params is None:
+            params = [torch.zeros(b, int(P), dtype=rdt, device=dev) for (_, _, P), b in zip(structures, batches)]
+        for (n, _, P), b, pa in zip(structures, batches, params):
+            if tuple(pa.shape) != (int(b), int(P)) or pa.dtype != rdt or not pa.is_contiguous():
+                raise ValueError("params[i] must be a contiguous [%d, %d] %s tensor" % (b, P, rdt))
+        self.params = list(params)
+        self.states = [torch.empty(b, 1 << n, dtype=cdt, device=dev) for n, b in self.sizes]
+        self.hams = list(hams) if hams is not None else [None] * k
+        if len(self.hams) != k:
+            raise ValueError("one Hamiltonian (or None) per structure")
+        opts = [dict(o) for o in options] if options is not None else [{} for _ in range(k)]
+        total = sum(b << n for n, b in self.sizes)
+        self.large = [(b << n) >= large_frac * total for n, b in self.sizes]
+
+        def build(olist: List[Dict[str, int]]) -> List[Program]:
+            progs = []
+            for (n, specs, P), o in zip(structures, olist):
+                prog = Program(n, specs, P, **{"fuse": 1, "jit": 1, **o})
+                prog.compile(dtype, 0)
+                progs.append(prog)
+            return progs
+
+        variants = {"default": build(opts)}
+        self.tuning_ms: Dict[str, Dict[str, float]] = {}
+        tuned = [dict(o) for o in opts]
+        if tune:
+            for i, ((n, specs, P), b) in enumerate(zip(structures, batches)):
+                if self.large[i] and not opts[i]:
+                    o, t = autotune_program(n, specs, P, b, dtype, jit=1)
+                    self.tuning_ms["%dq_x%d" % (n, b)] = t
+                    tuned[i] = o
+        self.tuned_options = tuned
+        if tuned != opts:
+            variants["tuned"] = build(tuned)
+        self._side = torch.cuda.Stream()
+        steps: Dict[str, Any] = {}
+        mixed = any(self.large) and not all(self.large)
+        for name, progs in variants.items():
+            steps["sequence_" + name] = self._sequence(progs)
+            steps["multi_" + name] = self._hybrid(progs, [False] * k, False)
+            if mixed:
+                steps["hybrid_" + name] = self._hybrid(progs, self.large, False)
+                steps["hybrid_streams_" + name] = self._hybrid(progs, self.large, True)
+        graphs = {name: CapturedStep(fn) for name, fn in steps.items()}
+        self.choice, self.candidate_ms = tune_step({name: g.replay for name, g in graphs.items()},
+                                                   "sequence_default", reps, margin)
+        # keep only the chosen step (the others' graphs hold their own memory)
+        self.step = steps[self.choice]
+        self.graph = graphs[self.choice]
+        self.programs = variants["tuned" if self.choice.endswith("tuned") else "default"]
+        self.multi_plan = getattr(self.step, "plan", None)
+
+    def _energy(self, i: int) -> Optional[torch.Tensor]:
+        h = self.hams[i]
+        return None if h is None else h.expect(self.states[i])
+
+    def _sequence(self, progs: List[Program]) -> Any:
+        def step():
+            for prog, pa, st in zip(progs, self.params, self.states):
+                prog.forward(pa, pa.shape[0], self.dtype, out=st)
+            return [self._energy(i) for i in range(len(progs))]
+        return step
+
+    def _hybrid(self, progs: List[Program], alone: List[bool], streams: bool) -> Any:
+        """programs with alone[i] on their own passes, the rest in one
+        multi-program launch per pass (on a second stream if ``streams``)"""
+        rest = [i for i, a in enumerate(alone) if not a]
+        plan = (MultiPlan([progs[i] for i in rest], self.dtype, batches=[self.sizes[i][1] for i in rest])
+                if rest else None)
+        items = [(progs[i], self.params[i], self.states[i]) for i in rest]
+
+        def run_rest(res):
+            if plan is not None:
+                plan.forward(items)
+            for i in rest:
+                res[i] = self._energy(i)
+
+        def step():
+            res: List[Optional[torch.Tensor]] = [None] * len(progs)
+            cur = torch.cuda.current_stream()
+            if streams:
+                self._side.wait_stream(cur)
+                with torch.cuda.stream(self._side):
+                    run_rest(res)
+            else:
+                run_rest(res)
+            for i, a in enumerate(alone):
+                if a:
+                    progs[i].forward(self.params[i], self.params[i].shape[0], self.dtype, out=self.states[i])
+                    res[i] = self._energy(i)
+            if streams:
+                cur.wait_stream(self._side)
+            return res
+        step.plan = plan
+        return step
+
+    def run(self) -> List[Optional[torch.Tensor]]:
+        """Replay the chosen step (one graph launch)."""
+        return self.graph.replay()
+
+    def run_eager(self) -> List[Optional[torch.Tensor]]:
+        """The chosen step without the graph (same kernels, launched one by one)."""
+        return self.step()

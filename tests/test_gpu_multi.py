@@ -2,7 +2,10 @@
 backends/pytorch_backend.py:605-617): compiled multi-program launches
 (engine.MultiPlan, tcm_forward_multi_plan) against single-program runs and
 the complex128 oracle at n <= 16; the program-level autotuner never picks a
-configuration slower than the default by its own measurement."""
+configuration slower than the default by its own measurement; the
+engine.UnevenBatch step (autotuned programs, step-level choice among the
+launch strategies, one HIP graph) against single-program runs and the
+oracle."""
 import numpy as np
 import pytest
 import torch
@@ -139,3 +142,52 @@ def test_circuit_api_routes_constant_circuits_to_matrix_cores(gpu):
     for o in ops[:10]:
         c2.unitary(*o.qubits, unitary=o.matrix()) if o.name != "cz" else c2.cz(*o.qubits)
     assert c2._mfma_program() is None
+
+
+def test_uneven_batch_step(gpu, tmp_path, monkeypatch):
+    """engine.UnevenBatch on four HEA structures (12 qubits x 96 parameter
+    sets holding most amplitudes, and 8, 9, 10 qubits x 2..5) with TFIM
+    energies: the step chosen among sequence / multi-program / hybrid /
+    two-stream hybrid candidates is not slower than the default by its own
+    measurement; replays of its HIP graph give every state of its program's
+    single run (4e-6) and of the oracle (4e-6 at complex64), the energies of
+    PauliSum.expect on those states (1e-9); parameters updated in place show
+    in the next replay; the eager step gives the same states."""
+    from tensorcircuit_amd import Circuit, engine, workloads
+    from tensorcircuit_amd.templates.blocks import hea_layers
+
+    monkeypatch.setenv("TCM_CACHE_DIR", str(tmp_path))
+    rng = np.random.default_rng(41)
+    sizes = [(12, 3, 96), (8, 2, 5), (9, 2, 3), (10, 3, 2)]
+    structs, hams = [], []
+    for n, L, B in sizes:
+        c = Circuit(n)
+        hea_layers(c, np.zeros((2 * L, n)), L, barrier=False)
+        specs, thetas = c._lower()
+        structs.append((n, specs, len(thetas)))
+        hams.append(engine.PauliSum(n, *workloads.tfim_terms(n)))
+    ub = engine.UnevenBatch(structs, [B for _, _, B in sizes], hams=hams)
+    assert ub.large == [True, False, False, False]
+    assert {"sequence_default", "multi_default", "hybrid_default", "hybrid_streams_default"} <= set(ub.candidate_ms)
+    assert ub.candidate_ms[ub.choice] <= ub.candidate_ms["sequence_default"] * (1 + 1e-9)
+
+    def check(es):
+        torch.cuda.synchronize()
+        for (n, L, B), prog, pa, st, h, e in zip(sizes, ub.programs, ub.params, ub.states, hams, es):
+            s1, _ = prog.forward(pa, B, "complex64")
+            np.testing.assert_allclose(np_of(st), np_of(s1), atol=4e-6, err_msg="n=%d" % n)
+            np.testing.assert_allclose(np_of(e), np_of(h.expect(s1)), atol=1e-9)
+            ref = sv.run_ops(n, sv.hea_ops(n, L), pa[B - 1].double().cpu().numpy())
+            np.testing.assert_allclose(np_of(st)[B - 1], ref, atol=4e-6, err_msg="oracle n=%d" % n)
+
+    for pa in ub.params:
+        pa.copy_(torch.as_tensor(rng.uniform(0, 2 * np.pi, size=tuple(pa.shape)).astype(np.float32)))
+    check(ub.run())
+    for pa in ub.params:
+        pa.add_(0.37)
+    check(ub.run())
+    replayed = [np_of(st).copy() for st in ub.states]
+    ub.run_eager()
+    torch.cuda.synchronize()
+    for st, r in zip(ub.states, replayed):
+        np.testing.assert_allclose(np_of(st), r, atol=1e-7)
