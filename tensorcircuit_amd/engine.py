@@ -1219,6 +1219,8 @@ class UnevenBatch:
         if len(self.hams) != k:
             raise ValueError("one Hamiltonian (or None) per structure")
         opts = [dict(o) for o in options] if options is not None else [{} for _ in range(k)]
+        if len(opts) != k:
+            raise ValueError("one options dict per structure")
         total = sum(b << n for n, b in self.sizes)
         self.large = [(b << n) >= large_frac * total for n, b in self.sizes]
 
