@@ -59,6 +59,8 @@ def parse():
                          "1 = one streaming matrix-core pass per block")
     ap.add_argument("--greedy", type=int, default=0,
                     help="1: first-fit pass scheduling (no search for fewer passes)")
+    ap.add_argument("--fusion", choices=["window", "first_fit"], default="window",
+                    help="c5: how gates are fused into 5-qubit blocks")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -384,7 +386,7 @@ def run_c5(args):
     n, depth = 30, 20
     t_stage = time.perf_counter()
     gates = workloads.brickwork(n, depth, 1005)
-    blocks = fusion.fuse(gates, n)
+    blocks = fusion.fuse(gates, n, method=args.fusion)
     p32 = engine.Program(n, blocks, 0, fuse=1, mfma=args.mfma, tile_bits=args.tile)
     staging = time.perf_counter() - t_stage
     st32 = torch.empty(1, 1 << n, dtype=torch.complex64, device="cuda")
@@ -405,7 +407,7 @@ def run_c5(args):
            "ms_per_step": round(dt32 * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "c64", "data": "synthetic",
            "config": {"workload": "brickwork_haar_su2_cz", "nqubits": n, "depth": depth, "gates": len(gates),
-                      "fused_blocks": nb, "seed": 1005, "staging_s": round(staging, 3),
+                      "fused_blocks": nb, "fusion": args.fusion, "seed": 1005, "staging_s": round(staging, 3),
                       "mfma_mode": args.mfma, "passes": p32.npasses},
            "fidelity_vs_c128": fid, "c128_ms": round(dt64 * 1e3, 3),
            "mfma": {"achieved": round(flops / dt32 / 1e12, 2), "peak": 157.3, "unit": "TFLOP/s",

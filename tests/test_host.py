@@ -217,24 +217,58 @@ def test_qir_round_trip():
     assert [q["name"] for q in d.to_qir()] == [q["name"] for q in c.to_qir()]
 
 
-def test_fusion_matches_unfused_oracle():
+def _rand_unitary(d, rng):
+    q, r = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+@pytest.mark.parametrize("method", ["window", "first_fit"])
+def test_fusion_matches_unfused_oracle(method):
     """fuse() multiplies gates into <=5-qubit dense blocks; replaying the
-    blocks through the oracle reproduces the unfused circuit."""
-    from tensorcircuit_amd import fusion, workloads
+    blocks through the oracle reproduces the unfused circuit (brickwork and
+    a random circuit with long-range two-qubit gates)."""
+    from tensorcircuit_amd import _lib, engine, fusion, workloads
 
     n = 9
-    gates = workloads.brickwork(n, 7, 11)
-    blocks = fusion.fuse(gates, n)
-    assert all(len(b.qubits) == 5 for b in blocks)
-    assert len(blocks) < len(gates) // 4
-    s1 = sv.zero_state(n)
-    for g in gates:
-        s1 = sv.apply_gate(s1, g.mat, g.qubits, n)
-    s2 = sv.zero_state(n)
-    for b in blocks:
-        np.testing.assert_allclose(b.mat.conj().T @ b.mat, np.eye(32), atol=1e-12)
-        s2 = sv.apply_gate(s2, b.mat, b.qubits, n)
-    np.testing.assert_allclose(s2, s1, atol=1e-12)
+    rng = np.random.default_rng(12)
+    rand = []
+    for _ in range(60):
+        if rng.random() < 0.5:
+            a, b = rng.choice(n, 2, replace=False)
+            rand.append(engine.GateSpec(_lib.G_MAT, (int(a), int(b)), -1, 0.0, 1.0, _rand_unitary(4, rng)))
+        else:
+            rand.append(engine.GateSpec(_lib.G_MAT, (int(rng.integers(n)),), -1, 0.0, 1.0, _rand_unitary(2, rng)))
+    for gates in (workloads.brickwork(n, 7, 11), rand):
+        blocks = fusion.fuse(gates, n, method=method)
+        assert all(len(b.qubits) == 5 for b in blocks)
+        assert len(blocks) < len(gates) // 4
+        s1 = sv.zero_state(n)
+        for g in gates:
+            s1 = sv.apply_gate(s1, g.mat, g.qubits, n)
+        s2 = sv.zero_state(n)
+        for b in blocks:
+            np.testing.assert_allclose(b.mat.conj().T @ b.mat, np.eye(32), atol=1e-12)
+            s2 = sv.apply_gate(s2, b.mat, b.qubits, n)
+        np.testing.assert_allclose(s2, s1, atol=1e-12)
+
+
+def test_fusion_window_needs_fewer_blocks():
+    """Choosing each block's qubits by the gates they absorb fuses the
+    30-qubit depth-20 brickwork into 50 blocks, against 76 for first fit;
+    gates wider than the block are rejected."""
+    from tensorcircuit_amd import _lib, engine, fusion, workloads
+
+    gates = workloads.brickwork(30, 20, 1005)
+    assert len(fusion.fuse(gates, 30, method="first_fit")) == 76
+    assert len(fusion.fuse(gates, 30)) == 50
+    for seed in (1, 2):
+        g = workloads.brickwork(16, 12, seed)
+        assert len(fusion.fuse(g, 16)) <= len(fusion.fuse(g, 16, method="first_fit"))
+    wide = engine.GateSpec(_lib.G_MAT, (0, 1, 2), -1, 0.0, 1.0, np.eye(8))
+    with pytest.raises(ValueError):
+        fusion.fuse([wide], 4, kmax=2)
+    with pytest.raises(ValueError):
+        fusion.fuse(gates, 30, method="nope")
 
 
 def test_brickwork_generator_matches_oracle():
@@ -359,20 +393,22 @@ def specs_qubits(specs, gi):
 
 
 def test_mfma_tile_pass_schedule():
-    """Matrix-core tile passes (mfma=2) group the 76 fused 5-qubit blocks of
-    the 30-qubit depth-20 brickwork into 34 passes of 2^13 amplitudes."""
+    """Matrix-core tile passes (mfma=2) group the 50 fused 5-qubit blocks of
+    the 30-qubit depth-20 brickwork into 22 passes of 2^13 amplitudes (the
+    76 first-fit blocks into 34)."""
     from tensorcircuit_amd import engine, fusion, workloads
 
     n = 30
-    blocks = fusion.fuse(workloads.brickwork(n, 20, 1005), n)
-    assert len(blocks) == 76
-    p1 = engine.Program(n, blocks, 0, fuse=1, mfma=1)
-    p2 = engine.Program(n, blocks, 0, fuse=1, mfma=2)
-    assert p1.npasses == 76
-    assert p2.npasses == 34
-    sched = p2.describe("complex64", 0)["sched"]
-    assert all(p["mfma"] == 1 and len(p["tile"]) == 13 for p in sched["passes"])
-    assert sorted(g for p in sched["passes"] for g in p["gates"]) == list(range(76))
+    for method, nb, npass in (("window", 50, 22), ("first_fit", 76, 34)):
+        blocks = fusion.fuse(workloads.brickwork(n, 20, 1005), n, method=method)
+        assert len(blocks) == nb
+        p1 = engine.Program(n, blocks, 0, fuse=1, mfma=1)
+        p2 = engine.Program(n, blocks, 0, fuse=1, mfma=2)
+        assert p1.npasses == nb
+        assert p2.npasses == npass
+        sched = p2.describe("complex64", 0)["sched"]
+        assert all(p["mfma"] == 1 and len(p["tile"]) == 13 for p in sched["passes"])
+        assert sorted(g for p in sched["passes"] for g in p["gates"]) == list(range(nb))
 
 
 def test_default_register_slots():
