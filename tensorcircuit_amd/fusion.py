@@ -141,8 +141,11 @@ def fuse(specs: Sequence[GateSpec], n: int, kmax: int = 5, pad: bool = True,
     for s in specs:
         if s.kind != _lib.G_MAT:
             raise ValueError("fuse() takes constant (MAT) gates only")
-        if len(s.qubits) > kmax:
-            raise ValueError("a gate on %d qubits does not fit a %d-qubit block" % (len(s.qubits), kmax))
+        qs = tuple(s.qubits)
+        if not qs or len(set(qs)) != len(qs) or min(qs) < 0 or max(qs) >= n:
+            raise ValueError("gate qubits %r: need distinct qubits in [0, %d)" % (qs, n))
+        if len(qs) > kmax:
+            raise ValueError("a gate on %d qubits does not fit a %d-qubit block" % (len(qs), kmax))
     qsets = [tuple(s.qubits) for s in specs]
     if method == "window":
         groups = _window(qsets, n, kmax)

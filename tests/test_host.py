@@ -255,7 +255,8 @@ def test_fusion_matches_unfused_oracle(method):
 def test_fusion_window_needs_fewer_blocks():
     """Choosing each block's qubits by the gates they absorb fuses the
     30-qubit depth-20 brickwork into 50 blocks, against 76 for first fit;
-    gates wider than the block are rejected."""
+    gates wider than the block, with repeated or out-of-range qubits or
+    with none are rejected."""
     from tensorcircuit_amd import _lib, engine, fusion, workloads
 
     gates = workloads.brickwork(30, 20, 1005)
@@ -267,6 +268,11 @@ def test_fusion_window_needs_fewer_blocks():
     wide = engine.GateSpec(_lib.G_MAT, (0, 1, 2), -1, 0.0, 1.0, np.eye(8))
     with pytest.raises(ValueError):
         fusion.fuse([wide], 4, kmax=2)
+    for qs in ((), (1, 1), (0, 4), (-1,)):
+        bad = engine.GateSpec(_lib.G_MAT, qs, -1, 0.0, 1.0, np.eye(2 ** len(qs)))
+        for method in ("window", "first_fit"):
+            with pytest.raises(ValueError):
+                fusion.fuse([bad], 4, method=method)
     with pytest.raises(ValueError):
         fusion.fuse(gates, 30, method="nope")
 
