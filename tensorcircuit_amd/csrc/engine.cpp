@@ -423,10 +423,36 @@ void pauli_tile_sched(tcm_pauli* H, int dt) {
     return;
   }
   S.ok = true;
-  const int k = std::min(n, dt == 0 ? 12 : 11);
-  const int Lw = std::min(dt == 0 ? 4 : 3, k);
   const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+  // launches of the first-fit packing below with k-bit tiles (a group too
+  // wide for a tile is a launch of its own)
+  auto npasses = [&](int k, uint64_t lmask) {
+    std::vector<uint64_t> rem;
+    int np = 0;
+    for (const DPGroup& G : H->groups)
+      if (__builtin_popcountll(G.flip | lmask) <= k) rem.push_back(G.flip);
+      else ++np;
+    while (!rem.empty()) {
+      uint64_t tm = (n <= k) ? all : lmask;
+      std::vector<uint64_t> rest;
+      for (uint64_t f : rem)
+        if (__builtin_popcountll(tm | f) <= k) tm |= f;
+        else rest.push_back(f);
+      rem.swap(rest);
+      ++np;
+    }
+    return np;
+  };
+  // complex64 tiles of 12 bits (32 KiB of LDS, several blocks per CU), or of
+  // 14 bits (128 KiB, one block of 1024 threads per CU) where that saves a
+  // pass: a 24-qubit sum of single X terms then takes 2 passes instead of 3
+  int k = std::min(n, dt == 0 ? 12 : 11);
+  const int Lw = std::min(dt == 0 ? 4 : 3, k);
   const uint64_t lmask = (1ull << Lw) - 1;
+  if (dt == 0 && n > k) {
+    const int k14 = std::min(n, 14);
+    if (npasses(k14, lmask) < npasses(k, lmask)) k = k14;
+  }
   std::vector<int> rem;
   for (int g = 0; g < (int)H->groups.size(); ++g) {
     if (__builtin_popcountll(H->groups[g].flip | lmask) > k) S.legacy.push_back(g);
@@ -467,7 +493,8 @@ void pauli_tile_sched(tcm_pauli* H, int dt) {
     }
     pp.g0 = (int)S.groups.size();
     pp.ng = (int)take.size();
-    pp.nthr_log = std::min(8, pp.k - 4);
+    // 16 elements per thread (up to 1024 threads)
+    pp.nthr_log = std::min(10, pp.k - 4);
     pp.slot0 = slot;
     slot += 1 << pp.nblk_log;
     for (int g : take) {

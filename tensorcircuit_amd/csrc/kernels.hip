@@ -1852,20 +1852,25 @@ void launch_pauli_tile(const DPPass& P, const DPPass* passes_dev, int pi, const 
                        bool energy, bool apply, hipStream_t st) {
   const int64_t blocks = batch << P.nblk_log;
   const size_t lds = sizeof(V2<T>) << P.k;
-  const dim3 grid((unsigned)blocks), block(1u << P.nthr_log);
+  // the kernel strides over the tile with blockDim.x threads (at most 256)
+  const dim3 grid((unsigned)blocks), block(1u << std::min(P.nthr_log, 8));
   const V2<T>* ps = (const V2<T>*)psi;
   const V2<T>* li = (const V2<T>*)lam_in;
   V2<T>* lo = (V2<T>*)lam_out;
   const int64_t ss = 1ll << n;
-  if (energy && apply)
+  if (energy && apply) {
+    ensure_lds_attr(k_pauli_tile<T, true, true>, lds);
     hipLaunchKernelGGL((k_pauli_tile<T, true, true>), grid, block, lds, st, passes_dev, pi, groups_dev, terms_dev,
                        runoff_dev, ps, li, lo, ss, alpha, epart, estride);
-  else if (energy)
+  } else if (energy) {
+    ensure_lds_attr(k_pauli_tile<T, true, false>, lds);
     hipLaunchKernelGGL((k_pauli_tile<T, true, false>), grid, block, lds, st, passes_dev, pi, groups_dev, terms_dev,
                        runoff_dev, ps, li, lo, ss, alpha, epart, estride);
-  else
+  } else {
+    ensure_lds_attr(k_pauli_tile<T, false, true>, lds);
     hipLaunchKernelGGL((k_pauli_tile<T, false, true>), grid, block, lds, st, passes_dev, pi, groups_dev, terms_dev,
                        runoff_dev, ps, li, lo, ss, alpha, epart, estride);
+  }
 }
 
 template <typename T>

@@ -481,6 +481,32 @@ def test_pauli_sums_vs_oracle(gpu, jit):
 
 
 @pytest.mark.parametrize("jit", [0, 1])
+def test_pauli_tfim_21q_wide_tiles(gpu, jit):
+    """A 21-qubit TFIM-like sum (ZZ chain with random weights, X on every
+    qubit): complex64 takes 14-bit tiles here (the 4 low bits plus 10 flip
+    bits per pass: 2 passes instead of 3), the second one of scattered high
+    bits; energies and H psi against the oracle, with a batch of 2."""
+    from tensorcircuit_amd import engine
+
+    rng = np.random.default_rng(21)
+    n = 21
+    terms = [(float(rng.uniform(-1, 1)), [3 if q in (i, i + 1) else 0 for q in range(n)]) for i in range(n - 1)]
+    terms += [(float(rng.uniform(-1, 1)), [1 if q == i else 0 for q in range(n)]) for i in range(n)]
+    ham = engine.PauliSum(n, [t[1] for t in terms], [t[0] for t in terms], jit=jit)
+    wsum = sum(abs(t[0]) for t in terms)
+    s = rng.standard_normal((2, 2**n)) + 1j * rng.standard_normal((2, 2**n))
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    for dt, tol in ((torch.complex64, 1e-5), (torch.complex128, 1e-12)):
+        st = torch.as_tensor(s).to(dt).cuda()
+        e = np_of(ham.expect(st))
+        hs = np_of(ham.apply(st))
+        for b in range(2):
+            ref = sv.pauli_sum_expectation(s[b], n, terms)
+            assert abs(e[b] - ref) < tol * wsum, (dt, b, e[b], ref)
+            np.testing.assert_allclose(hs[b], sv.pauli_sum_apply(s[b], n, terms), atol=tol * 10 * wsum)
+
+
+@pytest.mark.parametrize("jit", [0, 1])
 def test_pauli_wide_terms_and_tiles(gpu, jit):
     """Terms wider than an LDS tile (X on every qubit) run beside tiled ones."""
     from tensorcircuit_amd import engine
