@@ -59,7 +59,7 @@ def parse():
                          "1 = one streaming matrix-core pass per block")
     ap.add_argument("--greedy", type=int, default=0,
                     help="1: first-fit pass scheduling (no search for fewer passes)")
-    ap.add_argument("--fusion", choices=["window", "first_fit"], default="window",
+    ap.add_argument("--fusion", choices=["search", "window", "first_fit"], default="search",
                     help="c5: how gates are fused into 5-qubit blocks")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")

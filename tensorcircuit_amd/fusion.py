@@ -9,7 +9,8 @@ wants few, large, dense unitaries -- one HBM pass each.
 from __future__ import annotations
 
 import itertools
-from typing import List, Sequence, Tuple
+import random
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -58,7 +59,8 @@ def _first_fit(qsets: Sequence[Tuple[int, ...]], kmax: int) -> List[Tuple[List[i
     return out
 
 
-def _window(qsets: Sequence[Tuple[int, ...]], n: int, kmax: int) -> List[Tuple[List[int], List[int]]]:
+def _window(qsets: Sequence[Tuple[int, ...]], n: int, kmax: int,
+            rng: Optional[random.Random] = None) -> List[Tuple[List[int], List[int]]]:
     """Blocks chosen one at a time by the gates they absorb.  A gate is ready
     when every earlier gate on its qubits is in an emitted block.  For each
     ready gate the qubit set starts as its qubits and grows by one or two
@@ -67,7 +69,8 @@ def _window(qsets: Sequence[Tuple[int, ...]], n: int, kmax: int) -> List[Tuple[L
     closure absorbs the most is emitted as the next block.  A qubit set S
     absorbs every gate on qubits within S whose earlier gates are all
     emitted or absorbed.  Single-qubit gates weigh 1/100 of a two-qubit gate
-    (any block can take them)."""
+    (any block can take them).  With ``rng``, one block in five is the
+    second best instead (a randomised variant for the search below)."""
     queues: List[List[int]] = [[] for _ in range(n)]
     for i, qs in enumerate(qsets):
         for q in qs:
@@ -102,7 +105,7 @@ def _window(qsets: Sequence[Tuple[int, ...]], n: int, kmax: int) -> List[Tuple[L
     while left:
         ready = sorted({front(q, head) for q in range(n)} - {-1})
         ready = [g for g in ready if all(front(x, head) == g for x in qsets[g])]
-        best = None
+        best = second = None
         for g in ready:
             S = set(qsets[g])
             while len(S) < kmax:
@@ -122,22 +125,68 @@ def _window(qsets: Sequence[Tuple[int, ...]], n: int, kmax: int) -> List[Tuple[L
             taken, hd = absorb(S, head)
             w = weight(taken)
             if best is None or w > best[0]:
-                best = (w, S, taken, hd)
+                second, best = best, (w, S, taken, hd)
+            elif second is None or w > second[0]:
+                second = (w, S, taken, hd)
+        if rng is not None and second is not None and rng.random() < 0.2:
+            best = second
         _, S, taken, head = best
         left -= len(taken)
         out.append((sorted(S), taken))
     return out
 
 
+# Time of a matrix-core tile pass on a 30-qubit complex64 state: 1.52 ms per
+# pass plus 1.88 ms per block in it (k_mfma_tile<13> on MI355X, least squares
+# over the passes of the 30-qubit depth-20 brickwork).
+_PASS_MS, _BLOCK_MS = 1.52, 1.88
+
+
+def _search(qsets: Sequence[Tuple[int, ...]], n: int, kmax: int, tries: int,
+            seed: int) -> List[Tuple[List[int], List[int]]]:
+    """The window choice and ``tries`` - 1 seeded randomised variants of it,
+    each scored by the tile passes the matrix-core scheduler makes of its
+    blocks (``_PASS_MS`` per pass + ``_BLOCK_MS`` per block); the cheapest
+    wins, the first on ties.  The scheduler sees placeholder matrices: the
+    pass grouping depends on the blocks' qubits only."""
+    from .engine import Program
+
+    rng = random.Random(seed)
+    dummy = np.linalg.qr(np.random.default_rng(seed).normal(size=(32, 32)))[0].astype(np.complex128)
+    best = None
+    for t in range(max(1, tries)):
+        groups = _window(qsets, n, kmax, rng if t else None)
+        if kmax == 5 and n >= 10:
+            blocks = [GateSpec(_lib.G_MAT, tuple(_padded(S, n, kmax)), -1, 0.0, 1.0, dummy) for S, _ in groups]
+            cost = _PASS_MS * Program(n, blocks, 0, fuse=1, mfma=2).npasses + _BLOCK_MS * len(blocks)
+        else:
+            cost = float(len(groups))
+        if best is None or cost < best[0]:
+            best = (cost, groups)
+    return best[1]
+
+
+def _padded(qs: Sequence[int], n: int, kmax: int) -> List[int]:
+    """``qs`` widened to ``kmax`` qubits with the lowest unused ones."""
+    out = set(qs)
+    for q in range(n):
+        if len(out) >= kmax:
+            break
+        out.add(q)
+    return sorted(out)
+
+
 def fuse(specs: Sequence[GateSpec], n: int, kmax: int = 5, pad: bool = True,
-         method: str = "window") -> List[GateSpec]:
+         method: str = "window", tries: int = 64, seed: int = 0) -> List[GateSpec]:
     """Fusion of constant gates (kind MAT) into dense blocks of at most
     ``kmax`` qubits, in an order that keeps every gate after the gates it
     follows on shared qubits.  ``method``: "window" (default) picks each
-    block's qubit set by the number of gates it absorbs; "first_fit" sweeps
-    the gates in order and closes a block at the first gate that does not
-    fit.  With ``pad`` every block is widened to exactly ``kmax`` qubits
-    (identity on the extras)."""
+    block's qubit set by the number of gates it absorbs; "search" also tries
+    ``tries`` - 1 randomised variants of that choice (seeded by ``seed``) and
+    keeps the one whose blocks the matrix-core tile passes run fastest
+    (fewest blocks and passes); "first_fit" sweeps the gates in order and
+    closes a block at the first gate that does not fit.  With ``pad`` every
+    block is widened to exactly ``kmax`` qubits (identity on the extras)."""
     for s in specs:
         if s.kind != _lib.G_MAT:
             raise ValueError("fuse() takes constant (MAT) gates only")
@@ -149,19 +198,15 @@ def fuse(specs: Sequence[GateSpec], n: int, kmax: int = 5, pad: bool = True,
     qsets = [tuple(s.qubits) for s in specs]
     if method == "window":
         groups = _window(qsets, n, kmax)
+    elif method == "search":
+        groups = _search(qsets, n, kmax, tries, seed)
     elif method == "first_fit":
         groups = _first_fit(qsets, kmax)
     else:
         raise ValueError("unknown fusion method %r" % (method,))
     out: List[GateSpec] = []
     for qs_sorted, block in groups:
-        qs = set(qs_sorted)
-        if pad and len(qs) < kmax <= n:
-            for q in range(n):
-                if len(qs) >= kmax:
-                    break
-                qs.add(q)
-        order = sorted(qs)
+        order = _padded(qs_sorted, n, kmax) if pad and len(qs_sorted) < kmax <= n else sorted(qs_sorted)
         k = len(order)
         loc = {q: i for i, q in enumerate(order)}
         m = np.eye(2**k, dtype=np.complex128)

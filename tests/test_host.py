@@ -222,7 +222,7 @@ def _rand_unitary(d, rng):
     return q * (np.diag(r) / np.abs(np.diag(r)))
 
 
-@pytest.mark.parametrize("method", ["window", "first_fit"])
+@pytest.mark.parametrize("method", ["window", "search", "first_fit"])
 def test_fusion_matches_unfused_oracle(method):
     """fuse() multiplies gates into <=5-qubit dense blocks; replaying the
     blocks through the oracle reproduces the unfused circuit (brickwork and
@@ -239,7 +239,7 @@ def test_fusion_matches_unfused_oracle(method):
         else:
             rand.append(engine.GateSpec(_lib.G_MAT, (int(rng.integers(n)),), -1, 0.0, 1.0, _rand_unitary(2, rng)))
     for gates in (workloads.brickwork(n, 7, 11), rand):
-        blocks = fusion.fuse(gates, n, method=method)
+        blocks = fusion.fuse(gates, n, method=method, tries=6)
         assert all(len(b.qubits) == 5 for b in blocks)
         assert len(blocks) < len(gates) // 4
         s1 = sv.zero_state(n)
@@ -265,6 +265,15 @@ def test_fusion_window_needs_fewer_blocks():
     for seed in (1, 2):
         g = workloads.brickwork(16, 12, seed)
         assert len(fusion.fuse(g, 16)) <= len(fusion.fuse(g, 16, method="first_fit"))
+    # the seeded search keeps the window choice unless a variant's blocks
+    # take fewer tile passes and blocks (1.52 ms per pass + 1.88 ms per block)
+    def cost(b):
+        return 1.52 * engine.Program(30, b, 0, fuse=1, mfma=2).npasses + 1.88 * len(b)
+
+    win = fusion.fuse(gates, 30)
+    found = fusion.fuse(gates, 30, method="search", tries=16)
+    assert cost(found) <= cost(win) - 1.0
+    assert [b.qubits for b in fusion.fuse(gates, 30, method="search", tries=16)] == [b.qubits for b in found]
     wide = engine.GateSpec(_lib.G_MAT, (0, 1, 2), -1, 0.0, 1.0, np.eye(8))
     with pytest.raises(ValueError):
         fusion.fuse([wide], 4, kmax=2)
