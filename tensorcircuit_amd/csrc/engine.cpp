@@ -218,7 +218,7 @@ int64_t forward_ws(const Program& P, int dtype, int64_t batch) {
 // written by a tiled pass (nullable)
 template <typename T>
 int run_one_pass(Program& P, const Sched& S, const DeviceTables& D, int pi, const void* mtab, void* state,
-                 int64_t batch, void* ckout, hipStream_t st) {
+                 int64_t batch, void* ckout, hipStream_t st, bool zinit = false) {
   const int n = P.n;
   const bool mfma = S.k > 0 && S.dpass[pi].mfma;
   const bool tiled = S.k > 0 && S.dpass[pi].plain_op < 0 && !mfma;
@@ -232,7 +232,8 @@ int run_one_pass(Program& P, const Sched& S, const DeviceTables& D, int pi, cons
       return set_error(TCM_E_UNSUPPORTED, "matrix-core passes are complex64 only");
   } else if (tiled) {
     if (!S.jit.empty() && S.jit[pi].kernel >= 0) {
-      if (int rc = jit_launch_fwd(S, D, pi, sizeof(T) == 8, mtab, D.ctab, state, batch, n, ckout, st)) return rc;
+      if (int rc = jit_launch_fwd(S, D, pi, sizeof(T) == 8, mtab, D.ctab, state, batch, n, ckout, st, zinit ? 1 : 0))
+        return rc;
     } else if (S.R == 4) {
       launch_fused_fwd<T, 4>(S, D, pi, mtab, D.ctab, state, batch, n, ckout, st);
     } else {
@@ -245,13 +246,23 @@ int run_one_pass(Program& P, const Sched& S, const DeviceTables& D, int pi, cons
   return 0;
 }
 
+// forward passes; init_zero: the state starts as |0..0>, generated by a
+// compiled first pass itself (no separate write and read of the state)
+// unless the first pass is of another kind or the initial state is a
+// checkpoint
 template <typename T>
 int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const void* mtab, void* state,
-                       int64_t batch, void* ckpts, hipStream_t st) {
+                       int64_t batch, void* ckpts, hipStream_t st, bool init_zero = false) {
   const int n = P.n;
   const int64_t N = 1ll << n;
   const int64_t ck_stride = batch * N;
   auto ckpt_ptr = [&](size_t c) { return (void*)((char*)ckpts + (int64_t)c * ck_stride * 2 * sizeof(T)); };
+  bool ck0 = false;
+  if (ckpts)
+    for (int a : S.ckpt_after) ck0 = ck0 || a < 0;
+  const bool zfirst = init_zero && !ck0 && !S.passes.empty() && S.k > 0 && !S.dpass[0].mfma &&
+                      S.dpass[0].plain_op < 0 && !S.jit.empty() && S.jit[0].kernel >= 0;
+  if (init_zero && !zfirst) launch_init_zero<T>(state, n, batch, st);
   // checkpoints of the initial state (before any pass)
   if (ckpts)
     for (size_t c = 0; c < S.ckpt_after.size(); ++c)
@@ -268,7 +279,7 @@ int run_forward_passes(Program& P, const Sched& S, const DeviceTables& D, const 
           if (!ckout && tiled) ckout = ckpt_ptr(c);
           else extra.push_back(c);
         }
-    if (int rc = run_one_pass<T>(P, S, D, pi, mtab, state, batch, ckout, st)) return rc;
+    if (int rc = run_one_pass<T>(P, S, D, pi, mtab, state, batch, ckout, st, zfirst && pi == 0)) return rc;
     for (size_t c : extra) launch_copy<T>(ckpt_ptr(c), state, ck_stride, st);
   }
   return 0;
@@ -284,8 +295,7 @@ int forward_impl(Program& P, int dtype, const void* params, int64_t pstride, voi
   void* mtab = cv.take(batch * (int64_t)std::max(S.mtab_size, 1) * sizeof(T));
   if (cv.off > ws_bytes) return set_error(TCM_E_WORKSPACE, "workspace %lld < %lld bytes", (long long)ws_bytes, (long long)cv.off);
   launch_prep<T>(S, *D, D->ctab, params, pstride, mtab, batch, st);
-  if (init_zero) launch_init_zero<T>(state, P.n, batch, st);
-  if (int rc = run_forward_passes<T>(P, S, *D, mtab, state, batch, ckpts, st)) return rc;
+  if (int rc = run_forward_passes<T>(P, S, *D, mtab, state, batch, ckpts, st, init_zero != 0)) return rc;
   HIP_TRY(hipGetLastError(), "forward launch");
   return 0;
 }
@@ -650,8 +660,7 @@ int energy_grad_impl(Program& P, int dtype, const void* params, int64_t pstride,
   // forward with checkpoints
   launch_prep<T>(F, *DF, DF->ctab, params, pstride, w.mtab_f, batch, st);
   if (inputs) launch_copy<T>(w.psi, inputs, batch * N, st);
-  else launch_init_zero<T>(w.psi, P.n, batch, st);
-  if (int rc = run_forward_passes<T>(P, F, *DF, w.mtab_f, w.psi, batch, w.ckpt, st)) return rc;
+  if (int rc = run_forward_passes<T>(P, F, *DF, w.mtab_f, w.psi, batch, w.ckpt, st, inputs == nullptr)) return rc;
   // energy and lambda = H psi
   if (int rc = pauli_energy_apply<T>(H, dtype, w.psi, w.lam, nullptr, batch, w.epart, true, true, st)) return rc;
   launch_sum_slots(w.epart, (int64_t)pauli_slots(H, dtype) * 2, pauli_slots(H, dtype), 2, w.e2, batch, st);

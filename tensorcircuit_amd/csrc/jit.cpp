@@ -207,7 +207,7 @@ int jit_load(const Sched& S, DeviceTables& D) {
 }
 
 int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const void* mtab, const void* ctab,
-                   void* state, int64_t batch, int n, void* ckout, hipStream_t st) {
+                   void* state, int64_t batch, int n, void* ckout, hipStream_t st, int zinit) {
   const DPass& P = S.dpass[pi];
   const JitPass& J = S.jit[pi];
   (void)dt;
@@ -218,7 +218,8 @@ int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const 
   void* a_ck = ckout;
   long long total = batch << P.nblk_log;
   // one entry per parameter of the generated signature, in its order
-  void* args[] = {&a_st, &sstride, &a_mt, &mstride, &mbase, &dbase, &a_ct, &a_ck, &total};
+  int a_z = zinit;
+  void* args[] = {&a_st, &sstride, &a_mt, &mstride, &mbase, &dbase, &a_ct, &a_ck, &total, &a_z};
   const unsigned blocks = (unsigned)total;
   const hipError_t e = hipModuleLaunchKernel((hipFunction_t)D.jit_fn[J.kernel], blocks, 1, 1,
                                              1u << P.tthreads_log, 1, 1, 0, st, args, nullptr);

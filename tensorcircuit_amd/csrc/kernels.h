@@ -104,8 +104,9 @@ struct DeviceTables {
 
 // launch compiled pass kernels (jit.cpp)
 int jit_load(const Sched& S, DeviceTables& D);
+// zinit: the pass starts from |0..0> instead of reading `state`
 int jit_launch_fwd(const Sched& S, const DeviceTables& D, int pi, int dt, const void* mtab, const void* ctab,
-                   void* state, int64_t batch, int n, void* ckout, hipStream_t st);
+                   void* state, int64_t batch, int n, void* ckout, hipStream_t st, int zinit = 0);
 int jit_launch_adj(const Sched& S, const DeviceTables& D, int pi, int dt, const void* mtab, const void* ctab,
                    const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int64_t batch, int n,
                    double* part, int64_t pstride, hipStream_t st);

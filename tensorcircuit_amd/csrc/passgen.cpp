@@ -783,7 +783,19 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
   // forward pass: one tile
   if (!multi) G.line(fmt("__shared__ cT tile[%d];", 1 << k));
   G.line("cT* __restrict__ s = st + b * sstride + base;");
-  for (int it = 0; it < nit; ++it) G.line(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
+  if (multi) {
+    for (int it = 0; it < nit; ++it) G.line(fmt("const cT r%d = s[dlo + %lluull];", it, hi(it)));
+  } else {
+    // zinit: the pass starts from |0..0> instead of reading the state (the
+    // element index base + dlo + hi(it) is 0 only for it = 0, hi(0) = 0)
+    for (int it = 0; it < nit; ++it) G.line(fmt("cT r%d;", it));
+    G.line("if (zinit) {");
+    G.line("  r0 = cT{(base | (unsigned long long)dlo) == 0ull ? (T)1 : (T)0, (T)0};");
+    for (int it = 1; it < nit; ++it) G.line(fmt("  r%d = cT{(T)0, (T)0};", it));
+    G.line("} else {");
+    for (int it = 0; it < nit; ++it) G.line(fmt("  r%d = s[dlo + %lluull];", it, hi(it)));
+    G.line("}");
+  }
   for (int it = 0; it < nit; ++it) G.line(fmt("LDSB(tile, slo ^ %uu) = r%d;", swz_h((uint32_t)it << tl) * cb, it));
   G.line("__syncthreads();");
   for (int g = 0; g < dp.ngroups; ++g) {
@@ -851,7 +863,7 @@ std::string gen_pass_source(const Program& P, const Sched& S, int dt, int mode, 
         k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "
                 "tcm_fwd_k%d(cT* __restrict__ st, long long sstride, "
                 "const T* __restrict__ mtab, long long mstride, long long mbase, long long dbase, "
-                "const T* __restrict__ ct, cT* __restrict__ ck, long long total) {\n",
+                "const T* __restrict__ ct, cT* __restrict__ ck, long long total, int zinit) {\n",
                 NT, waves, id);
       else
         k = fmt("extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d))) "

@@ -299,7 +299,7 @@ def test_brickwork_generator_matches_oracle():
 
 def test_compiled_pass_signatures(tmp_path, monkeypatch):
     """Pass compiler on the CPU: the generated kernels declare the parameter
-    lists the launchers in jit.cpp pass (9 forward, 13 adjoint), and every
+    lists the launchers in jit.cpp pass (10 forward, 13 adjoint), and every
     tile pass of a small HEA program is compiled."""
     import re
 
@@ -315,7 +315,7 @@ def test_compiled_pass_signatures(tmp_path, monkeypatch):
     prog = engine.Program(n, specs, len(thetas), fuse=1, jit=1)
     assert prog.compile("complex64", 0) > 0.0
     assert prog.compile("complex64", 1) > 0.0
-    for mode, name, nparams in ((0, "tcm_fwd_k", 9), (1, "tcm_adj_k", 13)):
+    for mode, name, nparams in ((0, "tcm_fwd_k", 10), (1, "tcm_adj_k", 13)):
         srcs = "".join(p.read_text() for p in sorted(tmp_path.glob("tcm_passes_%d_*.hip" % mode)))
         sigs = re.findall(r"\b%s\d+\(([^)]*)\)" % name, srcs)
         assert sigs, mode
