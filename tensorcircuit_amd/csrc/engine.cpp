@@ -619,9 +619,12 @@ int reverse_sweep(Program& P, int dtype, const AdjWs& w, const void* psi_src, co
         lin = w.lam;
         continue;
       }
-      void* pout = c >= 0 ? nullptr : w.psi;
-      void* lout = (p == 0 && grad_input) ? grad_input : w.lam;
-      if (!A.jit.empty() && A.jit[p].kernel >= 0) {
+      // psi after the last pass (p = 0) is not read again; a compiled pass
+      // also skips lambda there unless it is the input gradient
+      const bool jitp = !A.jit.empty() && A.jit[p].kernel >= 0;
+      void* pout = (c >= 0 || p == 0) ? nullptr : w.psi;
+      void* lout = p > 0 ? w.lam : grad_input ? grad_input : jitp ? nullptr : w.lam;
+      if (jitp) {
         if (int rc = jit_launch_adj(A, *D, p, sizeof(T) == 8, w.mtab_a, D->ctab, pin, pout, lin, lout, batch, n,
                                     w.part, pstride, st))
           return rc;

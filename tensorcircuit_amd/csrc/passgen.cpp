@@ -764,7 +764,7 @@ std::string pass_body(const Program& P, const Sched& S, int pi, bool adj, int dt
     G.line("__syncthreads();");
     write_regs("l", prev, prev_where);
     G.line("__syncthreads();");
-    G.line("{ cT* so = lam_out + b * sstride + base;");
+    G.line("if (lam_out) { cT* so = lam_out + b * sstride + base;");
     for (int it = 0; it < nit; ++it)
       G.line(fmt("  so[dlo + %lluull] = LDSB(tb, slo ^ %uu);", hi(it), swz_h((uint32_t)it << tl) * cb));
     G.line("}");
