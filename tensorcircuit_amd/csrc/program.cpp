@@ -288,7 +288,7 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
                                       const std::function<uint64_t(int)>& mask,
                                       uint64_t fixed, int cap,
                                       const std::function<bool(int)>& solo,
-                                      std::vector<uint64_t>* used_out, bool climb) {
+                                      std::vector<uint64_t>* used_out, bool climb, int exact_max) {
   int maxid = 0;
   for (int it : items) maxid = std::max(maxid, it);
   std::vector<uint64_t> gmask(items.empty() ? 0 : maxid + 1, 0);
@@ -334,16 +334,16 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
   prepare(front[0]);
   std::vector<int> taken, rest;
   // exact search for a schedule one cluster shorter than the beam's (of up
-  // to 12 clusters), over the climbed set, its best neighbours and every
-  // window of `cap` consecutive free bits (a chain-structured circuit's
-  // staircase windows are such windows); bounded by a node budget of 4000
-  // per cluster (a 26-qubit 10-layer HEA: 11 -> 10 passes, found in under
-  // a second)
+  // to exact_max clusters), over the climbed set, its best neighbours and
+  // every window of `cap` consecutive free bits (a chain-structured
+  // circuit's staircase windows are such windows); bounded by a node budget
+  // (4000 for a target of up to 5 clusters, else 4000 per cluster: a
+  // 26-qubit 10-layer HEA goes from 11 to 10 passes in under a second)
   bool any_solo = false;
   for (int it : items) any_solo |= gsolo[it] != 0;
   auto shorter = [&](const Node& done) -> bool {
     const int target = (int)done.clusters.size() - 1;
-    if (!climb || any_solo || target < 1 || target > 12 || items.size() > 4000) return false;
+    if (!climb || any_solo || target < 1 || target > exact_max || items.size() > 4000) return false;
     uint64_t allbits = 0;
     for (int it : items) allbits |= gmask[it];
     allbits &= ~fixed;
@@ -356,7 +356,7 @@ std::vector<std::vector<int>> cluster(const std::vector<int>& items,
       for (int j = 0; j < cap; ++j) w |= 1ull << fb[i + j];
       windows.push_back(w);
     }
-    int budget = 4000 * target;
+    int budget = target <= 5 ? 4000 : 4000 * target;
     std::vector<std::vector<int>> cl;
     std::vector<uint64_t> sets;
     std::function<bool(const std::vector<int>&, int)> dfs = [&](const std::vector<int>& rem, int left) -> bool {
@@ -825,9 +825,12 @@ static void tile_run(const Program& P, Sched& S, const std::vector<int>& items, 
     clusters.push_back(items);
     used.push_back(0);
   } else {
+    // the longer exact search (schedules of up to 12 passes) where a pass
+    // moves a state of 2^22 amplitudes or more per item; 5 elsewhere, where
+    // the host time would outweigh a pass
     clusters = cluster(
         items, [&](int gi) { return gate_mask(P.gates[gi]); }, lmask, cap,
-        [](int) { return false; }, &used, !P.opts.greedy);
+        [](int) { return false; }, &used, !P.opts.greedy, n >= 22 ? 12 : 5);
   }
   for (size_t ci = 0; ci < clusters.size(); ++ci) {
     HPass hp;
@@ -856,7 +859,7 @@ static void tile_run(const Program& P, Sched& S, const std::vector<int>& items, 
     };
     std::vector<uint64_t> gused;
     auto groups = cluster(
-        hp.gates, lmask_of, 0, R, [&](int gi) { return P.gates[gi].nq >= 3; }, &gused, !P.opts.greedy);
+        hp.gates, lmask_of, 0, R, [&](int gi) { return P.gates[gi].nq >= 3; }, &gused, !P.opts.greedy, 5);
     for (size_t q = 0; q < groups.size(); ++q) {
       HGroup hg;
       hg.gates = groups[q];
@@ -890,7 +893,7 @@ static void mfma_tile_run(const Program& P, Sched& S, const std::vector<int>& it
   std::vector<uint64_t> used;
   auto clusters = cluster(
       items, [&](int gi) { return gate_mask(P.gates[gi]); }, lmask, cap, [](int) { return false; }, &used,
-      !P.opts.greedy);
+      !P.opts.greedy, 5);
   for (size_t ci = 0; ci < clusters.size(); ++ci) {
     HPass hp;
     hp.mfma = true;
